@@ -1,0 +1,201 @@
+"""Benchmark of the stencil hot path on one MI355X: prints ONE JSON line on stdout.
+
+Workload (the headline roofline configuration): 3D 7-point explicit FTCS heat, 1024^3 float32,
+periodic boundaries, initial state = sum of three sine modes (wavenumbers 1, 2, 3 per axis, unit
+amplitude) + 1e-3 * N(0,1) noise from seed 1, r = 0.15. One "step" is one sweep of the field
+(one lattice update of all 1024^3 sites) by the headline variant ``hip-cdna4/zmarch``; the other
+rungs (naive, tb<k>) are timed on the same field and reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--n 1024] [--variant hip-cdna4/zmarch]
+
+Timing: W untimed steps, then exactly K steps between two synchronisations; ``value`` is
+lattice updates per second over that region (GLUPS, inputs resident in HBM). The dominant
+kernel's average launch time comes from HIP events recorded on the stepper's own stream around
+the same region. ``cpu_baseline`` times the NumPy oracle (one core) on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+METRIC = ("solves/s + p99: batched server vs per-solve launches @1e-6; GLUPS, %HBM copy BW")
+PROFILES = ROOT / "profiles"
+
+
+def log(*a) -> None:
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_variant(key: str, u0: np.ndarray, stencil, steps: int, warmup: int) -> dict:
+    from pyheat3d_amd import _hip
+    from pyheat3d_amd.registry import VARIANTS, make_stepper
+
+    var = VARIANTS[key]
+    st = make_stepper(key, u0, stencil)
+    try:
+        k = var.steps_per_pass
+        steps = max(k, steps - steps % k)
+        st.step(max(warmup - warmup % k, k))
+        st.synchronize()
+        ev0, ev1 = _hip.Event(), _hip.Event()
+        t0 = time.perf_counter()
+        ev0.record(st.stream.handle)
+        st.step(steps)
+        ev1.record(st.stream.handle)
+        st.synchronize()
+        wall = time.perf_counter() - t0
+        ev_ms = ev0.elapsed_ms(ev1)
+        ev0.close()
+        ev1.close()
+        out = st.get()
+        mean = float(np.mean(out, dtype=np.float64))
+        finite = bool(np.isfinite(out).all())
+        del out
+    finally:
+        st.close()
+    cells = math.prod(u0.shape)
+    bpu = 2.0 * var.bytes_per_value / k
+    launches = steps // k
+    return {"key": key, "steps": steps, "k": k, "wall_s": wall, "event_ms": ev_ms,
+            "ms_per_step": 1e3 * wall / steps, "glups": cells * steps / wall / 1e9,
+            "kernel_avg_ms": ev_ms / launches, "bytes_per_update": bpu,
+            "algorithmic_bytes_per_launch": bpu * k * cells,
+            "effective_GBs": cells * steps / wall * bpu / 1e9, "final_mean": mean,
+            "finite": finite}
+
+
+def load_traffic(workload: str) -> dict | None:
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload."""
+    p = PROFILES / f"pmc_{workload}.json"
+    if not p.exists():
+        return None
+    d = json.loads(p.read_text())
+    return d
+
+
+def cpu_baseline(n: int, min_time: float) -> dict:
+    """NumPy oracle (periodic slicing stepper, one core) on an n^3 sample of the workload."""
+    from oracle import numpy_ref as nr
+    from pyheat3d_amd.analytic import sine_modes_periodic
+    from pyheat3d_amd.problem import heat_stencil
+
+    u = sine_modes_periodic(n, 3)
+    st_ = heat_stencil(3, 0.15, "periodic")
+    s = nr.NumpyStepper(u, st_.c0, st_.c1, "periodic")
+    s.step(1)  # warm-up
+    steps, t = 0, 0.0
+    while t < min_time:
+        t0 = time.perf_counter()
+        s.step(1)
+        t += time.perf_counter() - t0
+        steps += 1
+    return {"value": n ** 3 * steps / t / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"NumPy oracle periodic slicing stepper, {n}^3 fp32, {steps} steps "
+                      f"({t:.1f} s) on one core of {platform.processor() or 'host'} "
+                      f"({os.cpu_count()} cores visible)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--variant", default="hip-cdna4/zmarch")
+    ap.add_argument("--others", default="hip-cdna4/naive,hip-cdna4/tb2,hip-cdna4/tb3,hip-cdna4/tb4",
+                    help="comma-separated rungs timed beside the headline ('' for none)")
+    ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-min-time", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth probe")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(the multi-GPU z-slab design is documented, not built)", file=sys.stderr)
+        return 2
+
+    from pyheat3d_amd import _hip, analytic, roofline
+    from pyheat3d_amd.problem import heat_stencil
+    from pyheat3d_amd.registry import availability
+    from pyheat3d_amd.traffic import SPEC_BW_GBS
+
+    ok, why = availability(a.variant)
+    if not ok:
+        raise SystemExit(f"{a.variant} unavailable: {why}")
+    info = _hip.device_info(0)
+    log(f"device: {info['name']} {info['arch']} CUs={info['cu_count']}")
+
+    copy_kernel = copy_memcpy = None
+    if not a.no_copy:
+        copy_kernel = roofline.copy_bandwidth(1 << 31, method="kernel")
+        copy_memcpy = roofline.copy_bandwidth(1 << 31, method="memcpy")
+        log(f"copy bandwidth: kernel {copy_kernel:.0f} GB/s, hipMemcpy D2D {copy_memcpy:.0f} GB/s")
+
+    t0 = time.perf_counter()
+    u0 = analytic.sine_modes_periodic(a.n, 3)
+    log(f"init {a.n}^3 in {time.perf_counter() - t0:.1f} s")
+    stencil = heat_stencil(3, 0.15, "periodic")
+    init_mean = float(np.mean(u0, dtype=np.float64))
+
+    head = time_variant(a.variant, u0, stencil, a.steps, a.warmup)
+    log(f"{a.variant}: {head['glups']:.2f} GLUPS, {head['effective_GBs']:.0f} GB/s, "
+        f"kernel {head['kernel_avg_ms']:.3f} ms")
+    others = {}
+    for key in [x for x in a.others.split(",") if x]:
+        okk, whyk = availability(key)
+        if not okk:
+            others[key] = {"unavailable": whyk}
+            continue
+        r = time_variant(key, u0, stencil, a.steps, a.warmup)
+        log(f"{key}: {r['glups']:.2f} GLUPS ({r['effective_GBs']:.0f} GB/s minimal model)")
+        others[key] = {"glups": round(r["glups"], 3), "ms_per_step": round(r["ms_per_step"], 4),
+                       "minimal_GBs": round(r["effective_GBs"], 1), "k": r["k"],
+                       "single_step_equiv_GBs": round(r["glups"] * 8, 1),
+                       "mean_drift": abs(r["final_mean"] - init_mean) / abs(init_mean)}
+    del u0
+
+    achieved = head["algorithmic_bytes_per_launch"] / (head["kernel_avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    pmc = load_traffic(f"zmarch_{a.n}")
+    if pmc and pmc.get("variant") == a.variant:
+        traffic = pmc.get("hbm_bytes_per_launch")
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_n, a.cpu_min_time)
+
+    line = {
+        "metric": METRIC, "value": round(head["glups"], 3), "unit": "GLUPS", "n_gpus": 1,
+        "steps": head["steps"], "warmup": a.warmup, "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (3 sine modes + 1e-3 N(0,1) noise, seed 1)",
+        "config": {"workload": f"3D 7-point FTCS heat {a.n}^3 fp32 periodic r=0.15, "
+                               f"variant {a.variant}", "grid": [a.n] * 3, "r": 0.15,
+                   "variant": a.variant, "parallelism": "single GPU"},
+        "pct_copy_bw": None if not copy_kernel else round(100 * head["effective_GBs"] / copy_kernel, 1),
+        "copy_bw_GBs": {"kernel": copy_kernel and round(copy_kernel, 1),
+                        "hipMemcpyDtoD": copy_memcpy and round(copy_memcpy, 1)},
+        "pct_spec_8TBs": round(100 * head["effective_GBs"] / SPEC_BW_GBS, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": SPEC_BW_GBS,
+                     "unit": "GB/s", "frac": round(achieved / SPEC_BW_GBS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+        "check": {"finite": head["finite"],
+                  "periodic_mean_rel_drift": abs(head["final_mean"] - init_mean) / abs(init_mean)},
+        "rungs": others,
+        "device": {"name": info["name"], "arch": info["arch"], "cu_count": info["cu_count"]},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

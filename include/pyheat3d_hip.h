@@ -1,0 +1,120 @@
+/* pyheat3d_hip.h -- C ABI of the MI355X (gfx950) stencil library libpyheat3d_hip.so.
+ *
+ * The reference has no C ABI: its native boundary is CuPy's RawModule, which compiles
+ * src/pyheat3d/backends/stencil.cu with NVRTC and launches kernels with the argument tuple
+ * (u, o, [nz,] ny, nx, c0, c1) (backends/cupy_backend.py:86-100, 183-186). Each entry point
+ * below names the reference interface it replaces. Python binds them with ctypes
+ * (pyheat3d_amd/_hip.py); INTEGRATION.md shows the binding a reference maintainer would add.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative PH3_E* code; ph3_last_error() gives a
+ *     thread-local message for the last failure on the calling thread.
+ *   - Device pointers are plain addresses from ph3_malloc (or any hipMalloc'd memory).
+ *   - Streams and events are opaque handles (hipStream_t / hipEvent_t); a NULL stream is the
+ *     device's null stream.
+ *   - Arrays are C order (z, y, x) or (y, x), x contiguous, no padding, like the reference.
+ *   - storage: 0 = float32, 1 = float16, 2 = bfloat16 (bit pattern). Arithmetic is float32.
+ *   - Kernel launches are asynchronous; only the *_sync functions and ph3_memcpy block.
+ *   - Calls sharing one stream are not thread-safe against each other.
+ */
+#ifndef PYHEAT3D_HIP_H
+#define PYHEAT3D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PH3_OK 0
+#define PH3_EINVAL (-1)   /* bad argument (shape, storage, kind, alignment) */
+#define PH3_EHIP (-2)     /* a HIP runtime call failed */
+#define PH3_ENOMEM (-3)   /* device allocation failed */
+#define PH3_EUNSUP (-4)   /* valid request that this build does not support */
+
+/* sweep kinds (the reference's rungs, backends/cupy_backend.py:115-194) */
+#define PH3_NAIVE 0   /* one site per thread            (stencil.cu:43-93 naive2d/naive3d) */
+#define PH3_ZMARCH 1  /* 3D 2.5D z-march / 2D y-march   (stencil.cu:101-161 zmarch3d, tb2d K=1) */
+#define PH3_TB 2      /* k fused steps per pass         (stencil.cu:178-385 tb3d/tb2d) */
+
+/* ---- errors, version ------------------------------------------------------------------ */
+const char* ph3_last_error(void);
+int ph3_version(void);
+
+/* ---- device query: replaces the libcuda probe registry.cuda_available()
+ *      (src/pyheat3d/registry.py:164-176) and NVML machine_info (benchmark.py:103-117) ---- */
+int ph3_device_count(int* n);
+int ph3_device_info(int dev, char* name, int name_len, char* arch, int arch_len, int* cu_count,
+                    size_t* total_mem, int* wave_size, int* lds_per_cu, int* mem_clock_khz,
+                    int* bus_width_bits);
+int ph3_set_device(int dev);
+int ph3_device_sync(void);
+int ph3_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---- memory and copies: replace cp.asarray / dev.copy() / cp.asnumpy
+ *      (backends/cupy_backend.py:129-135, 196-200) ------------------------------------------ */
+int ph3_malloc(void** ptr, size_t bytes);
+int ph3_free(void* ptr);
+int ph3_host_alloc(void** ptr, size_t bytes);      /* pinned host memory */
+int ph3_host_free(void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device. Synchronous. */
+int ph3_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int ph3_memcpy_async(void* dst, const void* src, size_t bytes, int kind, void* stream);
+
+/* ---- streams and events: replace the default stream + deviceSynchronize
+ *      (cupy_backend.py:202-203) and the CUDA events of roofline.py:20-26 -------------------- */
+int ph3_stream_create(void** stream);
+int ph3_stream_destroy(void* stream);
+int ph3_stream_sync(void* stream);
+int ph3_event_create(void** event);
+int ph3_event_destroy(void* event);
+int ph3_event_record(void* event, void* stream);
+int ph3_event_sync(void* event);
+int ph3_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---- tile configuration (replaces the module-level tile dicts NAIVE*_BLOCK, ZMARCH, TB3D,
+ *      TB2D of cupy_backend.py:49-70 that bench/tune.py mutates) ----------------------------- */
+typedef struct ph3_tile {
+    int ty;    /* rows (y) per workgroup tile; 0 = default */
+    int zch;   /* planes (3D) or rows (2D) marched per workgroup; 0 = default (auto) */
+    int vec;   /* naive: sites per thread; 0 = default */
+    int waves; /* waves per workgroup (zmarch/tb); 0 = default */
+} ph3_tile;
+
+/* ---- the sweep: replaces CupyStepper._launch / step (cupy_backend.py:183-194) with the
+ *      kernels of stencil.cu. Advances u by k steps (kind PH3_TB) or 1 step (naive/zmarch)
+ *      into o. u and o must not overlap. b (float32, same shape, may be NULL) is added after
+ *      c0*u + c1*S in every step (the reference's NumPy source term, numpy_ref.py:44-45).
+ *      Dirichlet: the boundary layer of o is never written (callers keep it equal in both
+ *      ping-pong buffers). nz is ignored when dims == 2. ---------------------------------- */
+int ph3_sweep(int kind, int k, int storage, int periodic, int dims, int nz, int ny, int nx,
+              float c0, float c1, const void* u, void* o, const float* b,
+              const ph3_tile* tile, void* stream);
+
+/* Steps per pass supported by kind PH3_TB for (dims, storage); nonzero if supported. */
+int ph3_tb_supported(int dims, int k);
+
+/* ---- storage codec on the device: replaces to_bf16_bits / astype on the host
+ *      (cupy_backend.py:103-112, 138-142, 196-200) -------------------------------------------- */
+int ph3_encode(int storage, const float* src, void* dst, size_t n, void* stream);
+int ph3_decode(int storage, const void* src, float* dst, size_t n, void* stream);
+
+/* ---- measurement: replaces roofline.copy_bandwidth's torch copy_ (roofline.py:8-29) ---- */
+int ph3_copy_kernel(const void* src, void* dst, size_t bytes, void* stream);
+
+/* ---- FP64 Jacobi residual (no counterpart: the reference runs fixed sweep counts,
+ *      validation.py:89-110). Writes sum over updated sites of (b + c0*u + c1*S(u) - u)^2,
+ *      each value widened to double, into *out (device pointer). work must hold
+ *      ph3_residual_work_bytes() bytes of device memory. Deterministic: the summation order
+ *      depends only on the shape. ------------------------------------------------------- */
+size_t ph3_residual_work_bytes(void);
+int ph3_residual(int storage, int periodic, int dims, int nz, int ny, int nx, float c0,
+                 float c1, const void* u, const float* b, double* out, void* work,
+                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYHEAT3D_HIP_H */

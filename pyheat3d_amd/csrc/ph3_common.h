@@ -1,0 +1,90 @@
+// Shared device helpers for the pyheat3d HIP kernels (gfx950 / CDNA4, wave64).
+//
+// Every kernel computes the update in the canonical order of the reference
+// (src/pyheat3d/problem.py:3-10):
+//     s = (xm + xp); s = s + (ym + yp); [3D] s = s + (zm + zp);  v = (c0*u) + (c1*s); [v = v + b]
+// in float32, each operation separately rounded. Multiply-add contraction is disabled for the
+// whole library (-ffp-contract=off and the pragma below) so results equal the NumPy oracle bit
+// for bit. Storage may be float32, float16 or bfloat16; values are widened on load and rounded
+// to nearest even on store.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#pragma clang fp contract(off)
+
+namespace ph3 {
+
+enum Storage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
+
+// bfloat16 stored as its 16-bit pattern.
+struct bf16_t { uint16_t bits; };
+
+template <typename T> struct Codec;
+
+template <> struct Codec<float> {
+    static constexpr int code = kF32;
+    __device__ __forceinline__ static float load(float v) { return v; }
+    __device__ __forceinline__ static float store(float v) { return v; }
+};
+
+template <> struct Codec<_Float16> {
+    static constexpr int code = kF16;
+    __device__ __forceinline__ static float load(_Float16 v) { return (float)v; }
+    // v_cvt_f16_f32 rounds to nearest even in the default mode.
+    __device__ __forceinline__ static _Float16 store(float v) { return (_Float16)v; }
+};
+
+template <> struct Codec<bf16_t> {
+    static constexpr int code = kBF16;
+    __device__ __forceinline__ static float load(bf16_t v) {
+        return __uint_as_float((uint32_t)v.bits << 16);
+    }
+    // Round to nearest even on the float32 bit pattern (the bfloat16 definition used by the
+    // reference, backends/cupy_backend.py:103-107).
+    __device__ __forceinline__ static bf16_t store(float v) {
+        const uint32_t b = __float_as_uint(v);
+        bf16_t r;
+        r.bits = (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+        return r;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ float ld(const T* __restrict__ p, size_t i) { return Codec<T>::load(p[i]); }
+
+template <typename T>
+__device__ __forceinline__ void st(T* __restrict__ p, size_t i, float v) { p[i] = Codec<T>::store(v); }
+
+// The update with every operation rounded separately (contraction is off).
+__device__ __forceinline__ float update(float c0, float c1, float u, float s) {
+    const float a = c0 * u;
+    const float m = c1 * s;
+    return a + m;
+}
+
+__device__ __forceinline__ int wrap_lo(int i, int n) { return i < 0 ? i + n : i; }
+__device__ __forceinline__ int wrap_hi(int i, int n) { return i >= n ? i - n : i; }
+__device__ __forceinline__ int wrapi(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+}  // namespace ph3
+
+// Host-side error plumbing shared by every translation unit of the library.
+namespace ph3host {
+void set_error(const char* fmt, ...);
+int check(hipError_t e, const char* what);
+}  // namespace ph3host
+
+#define PH3_CHECK(call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return ph3host::check(e_, #call);                                \
+    } while (0)
+
+#define PH3_LAUNCH_CHECK(what)                                                                 \
+    do {                                                                                       \
+        hipError_t e_ = hipGetLastError();                                                     \
+        if (e_ != hipSuccess) return ph3host::check(e_, what);                                 \
+    } while (0)
