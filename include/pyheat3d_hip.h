@@ -76,11 +76,14 @@ int ph3_event_elapsed_ms(float* ms, void* start, void* stop);
 /* ---- tile configuration (replaces the module-level tile dicts NAIVE*_BLOCK, ZMARCH, TB3D,
  *      TB2D of cupy_backend.py:49-70 that bench/tune.py mutates) ----------------------------- */
 typedef struct ph3_tile {
-    int ty;    /* rows (y) per workgroup tile; 0 = default */
+    int ty;    /* zmarch/tb: rows (y) per wave; 0 = default */
     int zch;   /* planes (3D) or rows (2D) marched per workgroup; 0 = default (auto) */
-    int vec;   /* naive: sites per thread; 0 = default */
+    int vec;   /* sites per lane along x (naive: strided, zmarch: contiguous); 0 = default */
     int waves; /* waves per workgroup (zmarch/tb); 0 = default */
+    int flags; /* bit 0: non-temporal stores of the output; other bits must be 0 */
 } ph3_tile;
+
+#define PH3_TILE_NT_STORE 1
 
 /* ---- the sweep: replaces CupyStepper._launch / step (cupy_backend.py:183-194) with the
  *      kernels of stencil.cu. Advances u by k steps (kind PH3_TB) or 1 step (naive/zmarch)

@@ -24,9 +24,13 @@ class HipError(RuntimeError):
 
 
 class Tile(ctypes.Structure):
-    """``ph3_tile``: rows per tile, march chunk, sites per lane, waves (0 = default)."""
+    """``ph3_tile``: rows per wave, march chunk, sites per lane, waves, flags (0 = default)."""
 
-    _fields_ = [("ty", c_int), ("zch", c_int), ("vec", c_int), ("waves", c_int)]
+    _fields_ = [("ty", c_int), ("zch", c_int), ("vec", c_int), ("waves", c_int),
+                ("flags", c_int)]
+
+    def as_dict(self) -> dict[str, int]:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 # name -> (restype, argtypes); every symbol of include/pyheat3d_hip.h

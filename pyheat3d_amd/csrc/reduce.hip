@@ -86,7 +86,7 @@ extern "C" int ph3_residual(int storage, int periodic, int dims, int nz, int ny,
     }
     if (!u || !out || !work) { ph3host::set_error("residual: NULL pointer"); return PH3_EINVAL; }
     ph3::SweepArgs a{storage, periodic ? 1 : 0, dims, nz, ny, nx, c0, c1, u, nullptr, b,
-                     ph3_tile{0, 0, 0, 0}, (hipStream_t)stream};
+                     ph3_tile{0, 0, 0, 0, 0}, (hipStream_t)stream};
     double* partial = (double*)work;
     PH3_DISPATCH_TPS(a, {
         ph3::residual_pass1<T, P, S><<<ph3::kResBlocks, ph3::kResThreads, 0, a.stream>>>(

@@ -18,7 +18,7 @@ extern "C" int ph3_sweep(int kind, int k, int storage, int periodic, int dims, i
     if (u == o) { set_error("u and o must be distinct buffers"); return PH3_EINVAL; }
     if (storage < 0 || storage > 2) { set_error("storage %d not in {0,1,2}", storage); return PH3_EINVAL; }
     ph3::SweepArgs a{storage, periodic ? 1 : 0, dims, nz, ny, nx, c0, c1, u, o, b,
-                     tile ? *tile : ph3_tile{0, 0, 0, 0}, (hipStream_t)stream};
+                     tile ? *tile : ph3_tile{0, 0, 0, 0, 0}, (hipStream_t)stream};
     switch (kind) {
         case PH3_NAIVE: return ph3::launch_naive(a);
         case PH3_ZMARCH: return ph3::launch_zmarch(a);

@@ -1,19 +1,20 @@
 // Rung "zmarch": 2.5D blocking (counterpart of reference backends/stencil.cu:95-161 zmarch3d and
 // of the K = 1 "ymarch" case of tb2d, stencil.cu:283-385), redesigned for wave64.
 //
-// 3D. A wave owns an xy tile of 64*V sites (x) by R rows (y) and marches a chunk of z planes.
-// Lane l holds the V contiguous sites x0 + l*V .. x0 + l*V + V-1 of each row, so a row of the
-// tile is one 64*V*sizeof(T)-byte access per wave (16 B per lane when V*sizeof(T) = 16).
-// Three planes (z-1, z, z+1) of the tile sit in registers and plane z+2 is prefetched one
-// iteration ahead. Neighbours:
+// 3D. A workgroup of W waves owns an xy tile of 64*V sites (x) by W*R rows (y) and marches a
+// chunk of z planes. Wave w holds rows w*R .. w*R+R-1 of the tile; lane l holds the V
+// contiguous sites x0 + l*V .. x0 + l*V + V-1 of each of them, so a tile row is one
+// 64*V*sizeof(T)-byte access per wave (16 B per lane when V*sizeof(T) = 16). Three planes
+// (z-1, z, z+1) sit in registers and plane z+2 is prefetched one iteration ahead.
+// Neighbours:
 //   x: inside the lane, else from lane l-1 / l+1 by a cross-lane shuffle, else (wave edges,
 //      grid edges) from a halo value loaded one plane ahead;
-//   y: rows of the same lane, plus the rows just above and below the tile (2 extra row loads
-//      per plane, normally L2 hits: they are interior rows of the neighbouring waves);
+//   y: rows of the same lane; across waves, each wave publishes its first and last row of
+//      plane z in a double-buffered LDS slot (one barrier per plane); only the rows just above
+//      and below the whole workgroup tile are read from memory (by the first / last wave);
 //   z: the register queue.
-// There is no LDS and no barrier: waves are independent, so a workgroup of W waves is only a
-// scheduling unit (W tiles stacked in y). Workgroups are mapped XCD-aware: consecutive
-// logical tiles run on the same XCD, so halo rows shared by y-neighbours hit one L2.
+// Workgroups are mapped XCD-aware: consecutive logical tiles run on the same XCD, so the halo
+// rows shared by y-neighbouring workgroups hit one L2.
 //
 // 2D. A wave owns 64*V columns and marches a chunk of rows, R output rows per iteration from
 // a window of R+2 rows in registers (the same scheme with the y rows playing z).
@@ -44,12 +45,27 @@ __device__ __forceinline__ int logical_tile(int remap) {
     return (L & 7) * per + (L >> 3);
 }
 
+template <typename T, int V>
+__device__ __forceinline__ void store_out(T* __restrict__ o, size_t at, const float (&w)[V],
+                                          bool nt) {
+    if constexpr (V * sizeof(T) == 16 && sizeof(T) == 4) {
+        if (nt) {
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            f32x4 v = {w[0], w[1], w[2], w[3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(o + at));
+            return;
+        }
+    }
+    store_vec<T, V>(o, at, w);
+}
+
 template <typename T, bool P, bool S, int V, int R, int W>
 __global__ void __launch_bounds__(64 * W)
 zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
          int nz, int ny, int nx, float c0, float c1, int zch, int tiles_x, int tiles_y,
-         int remap)
+         int remap, int nt)
 {
+    __shared__ float edge[2][W][2][64 * V];  // [parity][wave][first/last row][site]
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int L = logical_tile(remap);
@@ -58,19 +74,20 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
     const int ty = rest % tiles_y;
     const int cz = rest / tiles_y;
     const int x = (tx * 64 + lane) * V;
-    const int j0 = (ty * W + wave) * R;
-    if (j0 >= ny) return;  // whole wave idle (no cross-wave communication exists)
+    const int J0 = ty * W * R;          // first row of the workgroup tile
+    const int j0 = J0 + wave * R;       // first row of this wave
     int z0 = cz * zch, z1 = min(z0 + zch, nz);
     if (!P) { z0 = max(z0, 1); z1 = min(z1, nz - 1); }
-    if (z0 >= z1) return;
+    if (z0 >= z1) return;               // uniform over the workgroup
 
     const bool xin = x < nx;
-    const int xl = xin ? x : nx - V;                 // load column (in bounds for idle lanes)
+    const int xl = xin ? x : nx - V;    // load column (in bounds for idle lanes)
     const bool useL = lane == 0;
     const bool useR = lane == 63 || x + V >= nx;
     const int hxl = P ? (xl == 0 ? nx - 1 : xl - 1) : max(xl - 1, 0);
     const int hxr = P ? (xl + V >= nx ? xl + V - nx : xl + V) : min(xl + V, nx - 1);
     const size_t plane = (size_t)nx * ny;
+    const bool top = wave == 0, bot = wave == W - 1;
 
     size_t roff[R];   // offset of row j0 + r within a plane (wrapped / clamped for loads)
     bool rstore[R];
@@ -81,8 +98,9 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
         roff[r] = (size_t)jl * nx;
         rstore[r] = j < ny && (P || (j > 0 && j < ny - 1));
     }
-    const size_t hup = (size_t)(P ? (j0 + ny - 1) % ny : max(j0 - 1, 0)) * nx;
-    const size_t hdn = (size_t)(P ? (j0 + R) % ny : min(j0 + R, ny - 1)) * nx;
+    // rows just above / below the workgroup tile (read by the first / last wave)
+    const size_t hup = (size_t)(P ? (J0 + ny - 1) % ny : max(J0 - 1, 0)) * nx;
+    const size_t hdn = (size_t)(P ? (J0 + W * R) % ny : min(J0 + W * R, ny - 1)) * nx;
 
     auto pidx = [&](int z) -> size_t {
         const int zz = P ? (z < 0 ? z + nz : (z >= nz ? z - nz : z)) : z;
@@ -90,7 +108,7 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
     };
 
     float cm[R][V], cc[R][V], cn[R][V];
-    float yu[V], yd[V];      // rows j0-1 and j0+R of plane z
+    float yu[V], yd[V];      // memory halo rows of plane z (first / last wave only)
     float xhl[R], xhr[R];    // x halos of plane z (used by edge lanes only)
     float bs[R][V];          // source of plane z
 
@@ -105,8 +123,10 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
             xhr[r] = useR ? ld(u, p0 + roff[r] + hxr) : 0.f;
             if (S) load_vec<float, V>(b, p0 + roff[r] + xl, bs[r]);
         }
-        load_vec<T, V>(u, p0 + hup + xl, yu);
-        load_vec<T, V>(u, p0 + hdn + xl, yd);
+#pragma unroll
+        for (int e = 0; e < V; ++e) yu[e] = yd[e] = 0.f;
+        if (top) load_vec<T, V>(u, p0 + hup + xl, yu);
+        if (bot) load_vec<T, V>(u, p0 + hdn + xl, yd);
     }
 
     for (int z = z0; z < z1; ++z) {
@@ -122,8 +142,27 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
                 nxr[r] = useR ? ld(u, p1 + roff[r] + hxr) : 0.f;
                 if (S) load_vec<float, V>(b, p1 + roff[r] + xl, nbs[r]);
             }
-            load_vec<T, V>(u, p1 + hup + xl, nyu);
-            load_vec<T, V>(u, p1 + hdn + xl, nyd);
+            if (top) load_vec<T, V>(u, p1 + hup + xl, nyu);
+            if (bot) load_vec<T, V>(u, p1 + hdn + xl, nyd);
+        }
+
+        // exchange the first / last rows of plane z between the waves of the workgroup
+        const int par = (z - z0) & 1;
+        if (W > 1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                edge[par][wave][0][lane * V + e] = cc[0][e];
+                edge[par][wave][1][lane * V + e] = cc[R - 1][e];
+            }
+            __syncthreads();
+            if (!top) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yu[e] = edge[par][wave - 1][1][lane * V + e];
+            }
+            if (!bot) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yd[e] = edge[par][wave + 1][0][lane * V + e];
+            }
         }
 
         const size_t pz = (size_t)z * plane;
@@ -145,7 +184,7 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
             if (xin && rstore[r]) {
                 const size_t at = pz + roff[r] + x;
                 if (P || (x > 0 && x + V < nx)) {
-                    store_vec<T, V>(o, at, w);
+                    store_out<T, V>(o, at, w, nt);
                 } else {
 #pragma unroll
                     for (int e = 0; e < V; ++e)
@@ -167,8 +206,14 @@ zmarch3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
                     if (S) bs[r][e] = nbs[r][e];
                 }
             }
+            if (top) {
 #pragma unroll
-            for (int e = 0; e < V; ++e) { yu[e] = nyu[e]; yd[e] = nyd[e]; }
+                for (int e = 0; e < V; ++e) yu[e] = nyu[e];
+            }
+            if (bot) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yd[e] = nyd[e];
+            }
         }
     }
 }
@@ -286,7 +331,7 @@ static int pick_vec(const SweepArgs& a, int elem_bytes) {
     return 1;
 }
 
-// Number of z chunks: enough workgroups for ~8 per CU on a 256-CU part, chunks >= 16 planes.
+// March chunk: enough workgroups for ~target per device, chunks of at least min_chunk.
 static int pick_chunk(int n_march, long tiles, int min_chunk, int target_groups) {
     long want = (target_groups + tiles - 1) / tiles;
     if (want < 1) want = 1;
@@ -300,13 +345,15 @@ static int zmarch_launch3(const SweepArgs& a) {
     const int tiles_x = (a.nx + 64 * V - 1) / (64 * V);
     const int tiles_y = (a.ny + R * W - 1) / (R * W);
     const long tiles = (long)tiles_x * tiles_y;
-    const int zch = a.tile.zch > 0 ? a.tile.zch : pick_chunk(a.nz, tiles, 16, 2048);
+    const int zch = a.tile.zch > 0 ? a.tile.zch : pick_chunk(a.nz, tiles, 64, 1024);
     const int chunks = (a.nz + zch - 1) / zch;
     const long groups = tiles * chunks;
     if (groups > 0x7fffffffL) { ph3host::set_error("zmarch: grid too large"); return PH3_EINVAL; }
     const int remap = (groups % 8) == 0 ? 1 : 0;
+    const int nt = (a.tile.flags & PH3_TILE_NT_STORE) ? 1 : 0;
     zmarch3d<T, P, S, V, R, W><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
-        (const T*)a.u, (T*)a.o, a.b, a.nz, a.ny, a.nx, a.c0, a.c1, zch, tiles_x, tiles_y, remap);
+        (const T*)a.u, (T*)a.o, a.b, a.nz, a.ny, a.nx, a.c0, a.c1, zch, tiles_x, tiles_y, remap,
+        nt);
     PH3_LAUNCH_CHECK("zmarch3d launch");
     return PH3_OK;
 }
@@ -325,34 +372,47 @@ static int ymarch_launch2(const SweepArgs& a) {
     return PH3_OK;
 }
 
+// (rows per wave, waves per workgroup) shapes built for the 3D kernel
 template <typename T, bool P, bool S, int V>
-static int zmarch_rows(const SweepArgs& a) {
-    const int ty = a.tile.ty ? a.tile.ty : 4;
-    if (a.dims == 3) {
-        switch (ty) {
-            case 1: return zmarch_launch3<T, P, S, V, 1, 4>(a);
-            case 2: return zmarch_launch3<T, P, S, V, 2, 4>(a);
-            case 4: return zmarch_launch3<T, P, S, V, 4, 4>(a);
-            case 8: return zmarch_launch3<T, P, S, V, 8, 4>(a);
-        }
-    } else {
-        switch (ty) {
-            case 1: return ymarch_launch2<T, P, S, V, 1, 4>(a);
-            case 2: return ymarch_launch2<T, P, S, V, 2, 4>(a);
-            case 4: return ymarch_launch2<T, P, S, V, 4, 4>(a);
-            case 8: return ymarch_launch2<T, P, S, V, 8, 4>(a);
-        }
-    }
-    ph3host::set_error("zmarch: rows per wave %d not in {1, 2, 4, 8}", ty);
+static int zmarch_shape3(const SweepArgs& a) {
+    const int R = a.tile.ty ? a.tile.ty : 4;
+    const int W = a.tile.waves ? a.tile.waves : 4;
+#define PH3_ZM(r, w) if (R == r && W == w) return zmarch_launch3<T, P, S, V, r, w>(a)
+    PH3_ZM(1, 16); PH3_ZM(2, 8); PH3_ZM(2, 4); PH3_ZM(4, 4); PH3_ZM(4, 8); PH3_ZM(8, 2);
+    PH3_ZM(4, 2); PH3_ZM(8, 1);
+#undef PH3_ZM
+    ph3host::set_error("zmarch3d: (rows %d, waves %d) not built; built: (1,16) (2,8) (2,4) "
+                       "(4,4) (4,8) (8,2) (4,2) (8,1)", R, W);
     return PH3_EINVAL;
 }
 
+template <typename T, bool P, bool S, int V>
+static int ymarch_shape2(const SweepArgs& a) {
+    const int R = a.tile.ty ? a.tile.ty : 4;
+    switch (R) {
+        case 1: return ymarch_launch2<T, P, S, V, 1, 4>(a);
+        case 2: return ymarch_launch2<T, P, S, V, 2, 4>(a);
+        case 4: return ymarch_launch2<T, P, S, V, 4, 4>(a);
+        case 8: return ymarch_launch2<T, P, S, V, 8, 4>(a);
+    }
+    ph3host::set_error("ymarch2d: rows per step %d not in {1, 2, 4, 8}", R);
+    return PH3_EINVAL;
+}
+
+template <typename T, bool P, bool S, int V>
+static int zmarch_dims(const SweepArgs& a) {
+    return a.dims == 3 ? zmarch_shape3<T, P, S, V>(a) : ymarch_shape2<T, P, S, V>(a);
+}
+
 int launch_zmarch(const SweepArgs& a) {
+    if (a.tile.flags & ~PH3_TILE_NT_STORE) {
+        ph3host::set_error("zmarch: unknown tile flags %#x", a.tile.flags);
+        return PH3_EINVAL;
+    }
     PH3_DISPATCH_TPS(a, {
         int v = a.tile.vec ? a.tile.vec : pick_vec(a, (int)sizeof(T));
-        if (a.nx % v) { ph3host::set_error("zmarch: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
-        if (v * (int)sizeof(T) > 16 || v < 1 || (v & (v - 1))) {
-            ph3host::set_error("zmarch: vec %d unsupported for this storage", v);
+        if (v < 1 || (v & (v - 1)) || v * (int)sizeof(T) > 16 || a.nx % v) {
+            ph3host::set_error("zmarch: vec %d unsupported for nx %d and this storage", v, a.nx);
             return PH3_EINVAL;
         }
         if (v > 1 && (((uintptr_t)a.u | (uintptr_t)a.o) % (size_t)(v * sizeof(T)) ||
@@ -360,19 +420,13 @@ int launch_zmarch(const SweepArgs& a) {
             ph3host::set_error("zmarch: buffers not aligned for vec %d", v);
             return PH3_EINVAL;
         }
-        if constexpr (sizeof(T) == 4) {
-            switch (v) {
-                case 1: return zmarch_rows<T, P, S, 1>(a);
-                case 2: return zmarch_rows<T, P, S, 2>(a);
-                case 4: return zmarch_rows<T, P, S, 4>(a);
-            }
-        } else {
-            switch (v) {
-                case 1: return zmarch_rows<T, P, S, 1>(a);
-                case 2: return zmarch_rows<T, P, S, 2>(a);
-                case 4: return zmarch_rows<T, P, S, 4>(a);
-                case 8: return zmarch_rows<T, P, S, 8>(a);
-            }
+        switch (v) {
+            case 1: return zmarch_dims<T, P, S, 1>(a);
+            case 2: return zmarch_dims<T, P, S, 2>(a);
+            case 4: return zmarch_dims<T, P, S, 4>(a);
+            case 8:
+                if constexpr (sizeof(T) == 2) return zmarch_dims<T, P, S, 8>(a);
+                break;
         }
         ph3host::set_error("zmarch: vec %d unsupported", v);
         return PH3_EINVAL;

@@ -44,11 +44,11 @@ def _run(key, u0, st, steps, source=None):
 def _check(key, u, ref):
     var = VARIANTS[key]
     err = nr.rel_err(u, ref)
-    if var.storage == "float32":
-        assert err <= 1e-6, (key, err)
-        assert np.array_equal(u, ref), (key, err, int(np.sum(u != ref)))
-    else:
-        assert err <= nr.TOLERANCE[var.storage], (key, err)
+    bound = 1e-6 if var.storage == "float32" else nr.TOLERANCE[var.storage]
+    assert err <= bound, (key, err)
+    # same per-site operations and the same round-to-nearest-even storage rounding: every
+    # storage type is expected to match its oracle bit for bit
+    assert np.array_equal(u, ref), (key, err, int(np.sum(u != ref)))
 
 
 @pytest.mark.parametrize("bc", ["dirichlet", "periodic"])
@@ -121,6 +121,36 @@ def test_odd_widths_and_vector_paths(key):
             st = heat_stencil(dims, 0.9 / (2 * dims), bc)
             ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 7)
             _check(key, _run(key, u0, st, 7), ref)
+
+
+ZMARCH_SHAPES = [(1, 16), (2, 8), (2, 4), (4, 4), (4, 8), (8, 2), (4, 2), (8, 1)]
+
+
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_zmarch_every_tile_shape(storage):
+    """Every built (rows, waves) shape, vector width and store flag equals the oracle."""
+    rng = np.random.default_rng(15)
+    key = "hip-cdna4/zmarch" + ("" if storage == "float32" else "-bf16")
+    for shape, bc in (((19, 37, 264), "periodic"), ((13, 70, 100), "dirichlet")):
+        u0 = rng.random(shape, dtype=np.float32)
+        st = heat_stencil(3, 0.15, bc)
+        ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 5, storage, 1)
+        vecs = (1, 2, 4) if storage == "float32" else (1, 2, 4, 8)
+        for (r, w) in ZMARCH_SHAPES:
+            for vec in vecs:
+                if shape[2] % vec:
+                    continue
+                for zch, flags in ((0, 0), (7, 1)):
+                    s = make_stepper(key, u0, st)
+                    s.tile_override = _hip.Tile(r, zch, vec, w, flags)
+                    print(f"[gpu] {key} {shape} {bc} rows={r} waves={w} vec={vec} zch={zch} "
+                          f"flags={flags}", flush=True)
+                    try:
+                        s.step(5)
+                        u = s.get()
+                    finally:
+                        s.close()
+                    assert np.array_equal(u, ref), (r, w, vec, zch, nr.rel_err(u, ref))
 
 
 def test_residual_matches_oracle():
