@@ -103,6 +103,14 @@ int ph3_tb_supported(int dims, int k);
 int ph3_encode(int storage, const float* src, void* dst, size_t n, void* stream);
 int ph3_decode(int storage, const void* src, float* dst, size_t n, void* stream);
 
+/* ---- synthetic initial states on the device (no counterpart: the reference draws fields on
+ *      the host with NumPy, benchmark.py:276-280). Site i of stream `seed` gets
+ *      (splitmix64(seed * 0xD1B54A32D192ED03 + i) >> 40) * 2^-24, rounded to storage.
+ *      kind 0: random inside, 0 on the outer layer; 1: random outer layer, 0 inside (Laplace
+ *      boundary data); 2: random everywhere. ---------------------------------------------- */
+int ph3_fill_random(int storage, void* dst, int dims, int nz, int ny, int nx,
+                    unsigned long long seed, int kind, void* stream);
+
 /* ---- measurement: replaces roofline.copy_bandwidth's torch copy_ (roofline.py:8-29) ---- */
 int ph3_copy_kernel(const void* src, void* dst, size_t bytes, void* stream);
 
@@ -115,6 +123,71 @@ size_t ph3_residual_work_bytes(void);
 int ph3_residual(int storage, int periodic, int dims, int nz, int ny, int nx, float c0,
                  float c1, const void* u, const float* b, double* out, void* work,
                  void* stream);
+
+/* ---- HIP graphs: capture whatever the calling thread enqueues on `stream` between begin and
+ *      end into an executable graph (no counterpart in the reference; its nearest analogue is
+ *      Warp's pre-recorded launches, backends/warp_backend.py:144-153). ---------------------- */
+int ph3_graph_begin(void* stream);
+int ph3_graph_end(void* stream, void** graph_exec);
+int ph3_graph_launch(void* graph_exec, void* stream);
+int ph3_graph_destroy(void* graph_exec);
+
+/* ---- batched solves: many independent uneven 2D/3D grids advanced together (no counterpart
+ *      in the reference, which times one grid at a time: benchmark.py:276-304).
+ *
+ * A solve is either a fixed number of steps (mode 0, heat) or Jacobi sweeps until the FP64
+ * residual falls below tol relative to the initial one (mode 1), checked every
+ * PH3_BATCH_CHECK sweeps. The state always lives in buf[done & 1]. Every decision (tier,
+ * tile shape, check points, reduction order) depends only on the solve itself, so a solve
+ * gives bit-identical results alone or in any batch.
+ *
+ * Tier 1 (grids of at most PH3_BATCH_SMALL_SITES sites): one workgroup owns the grid and runs
+ * its sweeps back to back with workgroup barriers; a persistent kernel hands grids to
+ * workgroups through an atomic counter. Tier 2 (larger grids): each pass is one varlen launch
+ * whose workgroups loop over the z-march tiles of every active large grid (a per-grid tile
+ * table), followed at each check point by a deterministic per-grid residual.
+ * ------------------------------------------------------------------------------------------ */
+#define PH3_BATCH_CHECK 64
+#define PH3_BATCH_SMALL_SITES 32768
+#define PH3_STALL_RING 32
+
+typedef struct ph3_solve_desc {
+    /* problem: set by the caller */
+    int dims, nz, ny, nx;        /* nz = 1 for 2D */
+    int periodic;
+    int storage;                 /* 0 float32, 1 float16, 2 bfloat16 */
+    float c0, c1;
+    void* buf[2];                /* ping-pong device buffers; the initial state is in buf[0] */
+    const float* src;            /* optional float32 source b, or NULL */
+    int mode;                    /* 0: run `steps` steps; 1: Jacobi to `tol` */
+    int stall_window;            /* mode 1: stagnation window in checks (0 = off, max 31) */
+    long long steps;             /* mode 0: steps; mode 1: sweep cap */
+    double tol;                  /* mode 1: target ||r|| / ||r0|| */
+    double log_rate;             /* mode 1: ln(Jacobi spectral radius) * PH3_BATCH_CHECK (< 0) */
+    double stall_fraction;       /* mode 1: required fraction of the predicted decrease */
+    /* state: zero-initialise; maintained by the library */
+    long long done;              /* steps / sweeps done */
+    double res0_sq;              /* FP64 squared residual of the initial state (mode 1) */
+    double res_sq;               /* latest FP64 squared residual (mode 1) */
+    int status;                  /* 0 running, 1 finished / converged, 2 stagnated, 3 capped */
+    int checks;                  /* residual checks done */
+    double ring[PH3_STALL_RING]; /* past residuals (squared) for the stagnation test */
+} ph3_solve_desc;
+
+typedef struct ph3_batch ph3_batch;
+
+size_t ph3_solve_desc_bytes(void);  /* sizeof(ph3_solve_desc), to check foreign mirrors */
+int ph3_batch_create(ph3_batch** batch, int capacity);
+int ph3_batch_destroy(ph3_batch* batch);
+/* Replace the batch's solve table with descs[0..n) (host memory) and compute the initial
+ * residual of every mode-1 solve whose `checks` is 0. Synchronous. */
+int ph3_batch_set(ph3_batch* batch, const ph3_solve_desc* descs, int n, void* stream);
+/* Enqueue one round: every running solve advances by up to `sweeps` steps (rounded up to a
+ * multiple of PH3_BATCH_CHECK for mode 1), stopping early when finished. Asynchronous and
+ * capturable into a graph (the launch shapes depend only on the table set last). */
+int ph3_batch_round(ph3_batch* batch, int sweeps, void* stream);
+/* Copy the solve table back to descs[0..n) (host). Synchronous on `stream`. */
+int ph3_batch_get(ph3_batch* batch, ph3_solve_desc* descs, int n, void* stream);
 
 #ifdef __cplusplus
 }

@@ -64,10 +64,62 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ph3_encode": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "ph3_decode": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "ph3_copy_kernel": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "ph3_fill_random": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_int, ctypes.c_ulonglong,
+                                c_int, c_void_p]),
     "ph3_residual_work_bytes": (c_size_t, []),
     "ph3_residual": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_float,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph3_graph_begin": (c_int, [c_void_p]),
+    "ph3_graph_end": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "ph3_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "ph3_graph_destroy": (c_int, [c_void_p]),
+    "ph3_solve_desc_bytes": (c_size_t, []),
+    "ph3_batch_create": (c_int, [POINTER(c_void_p), c_int]),
+    "ph3_batch_destroy": (c_int, [c_void_p]),
+    "ph3_batch_set": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "ph3_batch_round": (c_int, [c_void_p, c_int, c_void_p]),
+    "ph3_batch_get": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
 }
+
+BATCH_CHECK = 64          # PH3_BATCH_CHECK
+BATCH_SMALL_SITES = 32768  # PH3_BATCH_SMALL_SITES
+STALL_RING = 32           # PH3_STALL_RING
+
+
+class SolveDesc(ctypes.Structure):
+    """Mirror of ``ph3_solve_desc`` (size checked against the library at first use)."""
+
+    _fields_ = [
+        ("dims", c_int), ("nz", c_int), ("ny", c_int), ("nx", c_int),
+        ("periodic", c_int), ("storage", c_int), ("c0", c_float), ("c1", c_float),
+        ("buf", c_void_p * 2), ("src", c_void_p), ("mode", c_int), ("stall_window", c_int),
+        ("steps", ctypes.c_longlong), ("tol", c_double), ("log_rate", c_double),
+        ("stall_fraction", c_double), ("done", ctypes.c_longlong), ("res0_sq", c_double),
+        ("res_sq", c_double), ("status", c_int), ("checks", c_int),
+        ("ring", c_double * STALL_RING),
+    ]
+
+
+class Graph:
+    """An executable HIP graph captured from one stream."""
+
+    def __init__(self, stream: "Stream", record) -> None:
+        check(lib().ph3_graph_begin(stream.handle), "graph_begin")
+        try:
+            record()
+        finally:
+            p = c_void_p()
+            rc = lib().ph3_graph_end(stream.handle, ctypes.byref(p))
+        check(rc, "graph_end")
+        self.handle = int(p.value or 0)
+
+    def launch(self, stream: "Stream") -> None:
+        check(lib().ph3_graph_launch(self.handle, stream.handle), "graph_launch")
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().ph3_graph_destroy(self.handle), "graph_destroy")
+            self.handle = 0
 
 _lib: ctypes.CDLL | None = None
 

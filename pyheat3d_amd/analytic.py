@@ -176,3 +176,31 @@ def jacobi_eigen_iterate(n: int, sweeps: int, dims: int = 3) -> float:
     lam = 4.0 * dims * math.sin(0.5 * math.pi * h) ** 2 / (h * h)
     alpha = dims * math.pi ** 2 / lam
     return alpha * (1.0 - math.cos(math.pi * h) ** sweeps)
+
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix_field(shape: tuple[int, ...], seed: int, kind: int = 2) -> np.ndarray:
+    """Host mirror of ``ph3_fill_random``: the same float32 field, bit for bit.
+
+    Site ``i`` (C order) gets ``(splitmix64(seed * 0xD1B54A32D192ED03 + i) >> 40) * 2^-24``.
+    ``kind`` 0: random inside and 0 on the outer layer (heat initial state); 1: random outer
+    layer and 0 inside (Laplace boundary data, zero initial guess); 2: random everywhere.
+    """
+    n = int(np.prod(shape))
+    z = np.arange(n, dtype=np.uint64) + np.uint64((seed * 0xD1B54A32D192ED03) & _M64)
+    z += np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    v = (z >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
+    v = v.reshape(shape)
+    if kind in (0, 1):
+        edge = np.zeros(shape, dtype=bool)
+        for ax in range(len(shape)):
+            idx = [slice(None)] * len(shape)
+            idx[ax] = [0, -1]
+            edge[tuple(idx)] = True
+        v[edge if kind == 0 else ~edge] = 0.0
+    return v

@@ -1,0 +1,540 @@
+// Batched solves: many independent uneven 2D/3D grids advanced together (see the ph3_batch_*
+// section of include/pyheat3d_hip.h for the contract).
+//
+// Tier 1, grids of at most PH3_BATCH_SMALL_SITES sites: one workgroup owns a grid and runs its
+// sweeps back to back, separated only by workgroup barriers (the grid stays in L2); the FP64
+// residual is a workgroup reduction in a fixed order. A persistent kernel hands grids to
+// workgroups through an atomic counter, so a single launch advances every small grid.
+// Tier 2, larger grids: one varlen launch per pass whose workgroups loop over the z-march (3D)
+// or y-march (2D) tiles of every running grid of one (storage, dims, bc, source) class, found by
+// binary search in a per-class tile prefix table. Every PH3_BATCH_CHECK passes three small
+// kernels advance the step counters, compute the residual of each Jacobi grid in fixed slices
+// and apply the convergence / stagnation / cap test.
+// All per-site arithmetic is the canonical update of ph3_common.h, so a grid gets the same bits
+// whichever tier, tile or batch it runs in; tier and check points depend only on the grid.
+#include "march_tiles.h"
+
+#include <math.h>
+#include <vector>
+
+using ph3host::set_error;
+
+namespace ph3 {
+
+constexpr int kWG = 256;          // lanes per workgroup in every batch kernel
+constexpr int kPassGroups = 2048; // workgroups of the persistent / looping launches
+constexpr int kSlices = 16;       // residual slices per tier-2 grid
+constexpr int kZch3 = 32;         // tier-2 3D z chunk
+constexpr int kYch2 = 64;         // tier-2 2D y chunk
+constexpr int kR3 = 4, kW3 = 4;   // tier-2 3D tile: 4 waves x 4 rows, 64 sites wide
+constexpr int kR2 = 4, kW2 = 4;   // tier-2 2D tile: 4 waves x 64 columns, 4 rows per step
+
+__device__ __forceinline__ long long sites_of(const ph3_solve_desc& d) {
+    return (long long)d.nz * d.ny * d.nx;
+}
+
+// ---- one workgroup: sweep and residual of a whole grid ---------------------------------------
+
+template <typename T>
+__device__ void wg_sweep(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
+                         int dims, int nz, int ny, int nx, bool P, float c0, float c1) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t plane = (size_t)nx * ny;
+    const int rows = nz * ny;
+    for (int row = wave; row < rows; row += kWG / 64) {
+        const int k = row / ny, j = row - k * ny;
+        if (!P && (j == 0 || j == ny - 1 || (dims == 3 && (k == 0 || k == nz - 1)))) continue;
+        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        const size_t base = (size_t)k * plane, r0 = base + (size_t)j * nx;
+        const size_t rm = base + (size_t)jm * nx, rp = base + (size_t)jp * nx;
+        size_t zm = 0, zp = 0;
+        if (dims == 3) {
+            const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+            zm = (size_t)km * plane + (size_t)j * nx;
+            zp = (size_t)kp * plane + (size_t)j * nx;
+        }
+        for (int i = lane; i < nx; i += 64) {
+            if (!P && (i == 0 || i == nx - 1)) continue;
+            const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+            float s = ld(u, r0 + im) + ld(u, r0 + ip);
+            s = s + (ld(u, rm + i) + ld(u, rp + i));
+            if (dims == 3) s = s + (ld(u, zm + i) + ld(u, zp + i));
+            float w = update(c0, c1, ld(u, r0 + i), s);
+            if (b) w = w + b[r0 + i];
+            st(o, r0 + i, w);
+        }
+    }
+}
+
+// Sum over rows row0, row0 + rstride, ... of the squared defect, per lane, in FP64.
+template <typename T>
+__device__ double residual_rows(const T* __restrict__ u, const float* __restrict__ b, int dims,
+                                int nz, int ny, int nx, bool P, float c0, float c1, int row0,
+                                int rstride, int lane0, int lstride) {
+    const double c0d = c0, c1d = c1;
+    const size_t plane = (size_t)nx * ny;
+    const int rows = nz * ny;
+    double acc = 0.0;
+    for (int row = row0; row < rows; row += rstride) {
+        const int k = row / ny, j = row - k * ny;
+        if (!P && (j == 0 || j == ny - 1 || (dims == 3 && (k == 0 || k == nz - 1)))) continue;
+        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        const size_t base = (size_t)k * plane, r0 = base + (size_t)j * nx;
+        for (int i = lane0; i < nx; i += lstride) {
+            if (!P && (i == 0 || i == nx - 1)) continue;
+            const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+            double s = (double)ld(u, r0 + im) + (double)ld(u, r0 + ip);
+            s += (double)ld(u, base + (size_t)jm * nx + i) + (double)ld(u, base + (size_t)jp * nx + i);
+            if (dims == 3) {
+                const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+                s += (double)ld(u, (size_t)km * plane + (size_t)j * nx + i) +
+                     (double)ld(u, (size_t)kp * plane + (size_t)j * nx + i);
+            }
+            const double uc = (double)ld(u, r0 + i);
+            double r = c0d * uc + c1d * s - uc;
+            if (b) r += (double)b[r0 + i];
+            acc += r * r;
+        }
+    }
+    return acc;
+}
+
+// Fixed-order workgroup sum (every lane gets the result).
+__device__ double wg_sum(double v, double* sm) {
+    sm[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int w = kWG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double t = sm[0];
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+__device__ double wg_residual(const ph3_solve_desc& d, const void* u, double* sm) {
+    const double part = residual_rows<T>((const T*)u, d.src, d.dims, d.nz, d.ny, d.nx,
+                                         d.periodic != 0, d.c0, d.c1, threadIdx.x >> 6, kWG / 64,
+                                         threadIdx.x & 63, 64);
+    return wg_sum(part, sm);
+}
+
+// Convergence / stagnation / cap test after check number `checks` (1-based).
+__device__ void apply_check(ph3_solve_desc& d, double r) {
+    const int c = d.checks + 1;
+    d.checks = c;
+    d.res_sq = r;
+    const double rel = d.res0_sq > 0.0 ? sqrt(r / d.res0_sq) : 0.0;
+    int status = 0;
+    if (rel <= d.tol) {
+        status = 1;
+    } else {
+        if (d.stall_window > 0 && c >= d.stall_window && d.log_rate < 0.0) {
+            const double old = d.ring[(c - d.stall_window) % PH3_STALL_RING];
+            const double drop = 0.5 * log(fmax(r, 1e-300) / fmax(old, 1e-300));
+            if (drop > d.stall_fraction * d.stall_window * d.log_rate) status = 2;
+        }
+        if (status == 0 && d.done >= d.steps) status = 3;
+    }
+    d.ring[c % PH3_STALL_RING] = r;
+    d.status = status;
+}
+
+// ---- tier 1 ----------------------------------------------------------------------------------
+
+template <typename T>
+__device__ void small_solve(ph3_solve_desc* dp, int sweeps, double* sm) {
+    __shared__ long long s_done;
+    __shared__ int s_status;
+    if (threadIdx.x == 0) { s_done = dp->done; s_status = dp->status; }
+    __syncthreads();
+    long long done = s_done;
+    int status = s_status;
+    const ph3_solve_desc d = *dp;  // read-only copy of the problem fields
+    if (status != 0) return;
+    const long long target = done + sweeps;
+    while (status == 0 && done < target) {
+        long long seg = d.mode == 1 ? PH3_BATCH_CHECK : target - done;
+        if (seg > d.steps - done) seg = d.steps - done;
+        if (seg <= 0) {  // nothing left to run: finished (heat) or at the sweep cap (Jacobi)
+            status = d.mode == 0 ? 1 : 3;
+            if (threadIdx.x == 0) dp->status = status;
+            break;
+        }
+        for (long long i = 0; i < seg; ++i) {
+            const long long cur = (done + i) & 1;
+            wg_sweep<T>((const T*)d.buf[cur], (T*)d.buf[cur ^ 1], d.src, d.dims, d.nz, d.ny, d.nx,
+                        d.periodic != 0, d.c0, d.c1);
+            __syncthreads();
+        }
+        done += seg;
+        if (d.mode == 0) {
+            if (done >= d.steps) status = 1;
+        } else {
+            const double r = wg_residual<T>(d, d.buf[done & 1], sm);
+            if (threadIdx.x == 0) {
+                dp->done = done;
+                apply_check(*dp, r);
+                s_status = dp->status;
+            }
+            __syncthreads();
+            status = s_status;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        dp->done = done;
+        if (d.mode == 0) dp->status = status;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+batch_small(ph3_solve_desc* descs, const int* __restrict__ list, int n, int* counter, int sweeps) {
+    __shared__ double sm[kWG];
+    __shared__ int s_idx;
+    for (;;) {
+        if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = s_idx;
+        __syncthreads();
+        if (t >= n) return;
+        small_solve<T>(descs + list[t], sweeps, sm);
+        __syncthreads();
+    }
+}
+
+__global__ void batch_reset(int* counter) { *counter = 0; }
+
+// Initial residual of every listed Jacobi solve that has not been checked yet (one workgroup per
+// solve, the same reduction as tier 1).
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) {
+    __shared__ double sm[kWG];
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        ph3_solve_desc* dp = descs + list[t];
+        const ph3_solve_desc d = *dp;
+        if (d.mode != 1 || d.checks != 0) continue;   // uniform over the workgroup
+        const double r = wg_residual<T>(d, d.buf[d.done & 1], sm);
+        if (threadIdx.x == 0) {
+            dp->res0_sq = r;
+            dp->res_sq = r;
+            dp->ring[0] = r;
+            if (r == 0.0) dp->status = 1;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- tier 2 ----------------------------------------------------------------------------------
+
+__host__ __device__ inline long long tiles_of(int dims, int nz, int ny, int nx) {
+    if (dims == 3) {
+        const long long tx = (nx + 63) / 64, ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
+        return tx * ty * ((nz + kZch3 - 1) / kZch3);
+    }
+    const long long strips = (nx + 63) / 64;
+    return ((strips + kW2 - 1) / kW2) * ((ny + kYch2 - 1) / kYch2);
+}
+
+template <typename T, bool P, bool S, int DIMS>
+__global__ void __launch_bounds__(kWG)
+batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
+           const long long* __restrict__ prefix, int n, int p) {
+    const long long total = prefix[n];
+    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = n - 1;  // last i with prefix[i] <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const ph3_solve_desc& d = descs[list[lo]];
+        if (d.status == 0 && d.done + p < d.steps) {
+            const long long local = t - prefix[lo];
+            const int cur = (int)((d.done + p) & 1);
+            const T* u = (const T*)d.buf[cur];
+            T* o = (T*)d.buf[cur ^ 1];
+            if (DIMS == 3) {
+                const int tx = (d.nx + 63) / 64, ty = (d.ny + kR3 * kW3 - 1) / (kR3 * kW3);
+                const int ix = (int)(local % tx), rest = (int)(local / tx);
+                zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0, d.c1,
+                                                    kZch3, ix, rest % ty, rest / ty, 0);
+            } else {
+                const int gxn = ((d.nx + 63) / 64 + kW2 - 1) / kW2;
+                ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1, kYch2,
+                                                    (int)(local % gxn), (int)(local / gxn));
+            }
+        }
+        __syncthreads();  // the 3D tile's LDS slot is reused by the next tile
+    }
+}
+
+// After a segment of PH3_BATCH_CHECK passes: advance the step counters of running solves.
+__global__ void batch_advance(ph3_solve_desc* descs, const int* __restrict__ list, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    ph3_solve_desc& d = descs[list[t]];
+    if (d.status != 0) return;
+    long long adv = d.steps - d.done;
+    if (adv > PH3_BATCH_CHECK) adv = PH3_BATCH_CHECK;
+    d.done += adv;
+    if (d.mode == 0 && d.done >= d.steps) d.status = 1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
+               double* __restrict__ partial) {
+    __shared__ double sm[kWG];
+    for (int item = blockIdx.x; item < n * kSlices; item += gridDim.x) {
+        const int t = item / kSlices, sl = item - t * kSlices;
+        const ph3_solve_desc& d = descs[list[t]];
+        if (d.mode != 1 || d.status != 0 || d.storage != Codec<T>::code) continue;
+        const double part = residual_rows<T>((const T*)d.buf[d.done & 1], d.src, d.dims, d.nz,
+                                             d.ny, d.nx, d.periodic != 0, d.c0, d.c1,
+                                             sl * (kWG / 64) + (threadIdx.x >> 6),
+                                             kSlices * (kWG / 64), threadIdx.x & 63, 64);
+        const double s = wg_sum(part, sm);
+        if (threadIdx.x == 0) partial[item] = s;
+    }
+}
+
+__global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list, int n,
+                            const double* __restrict__ partial) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    ph3_solve_desc& d = descs[list[t]];
+    if (d.mode != 1 || d.status != 0) return;
+    double r = 0.0;
+    for (int s = 0; s < kSlices; ++s) r += partial[t * kSlices + s];
+    apply_check(d, r);
+}
+
+}  // namespace ph3
+
+// ---- host side -----------------------------------------------------------------------------
+
+struct Tier2Class {
+    int storage, dims, periodic, src;
+    std::vector<int> idx;
+    int* d_list = nullptr;
+    long long* d_prefix = nullptr;
+};
+
+struct ph3_batch {
+    int cap = 0, n = 0;
+    ph3_solve_desc* d_desc = nullptr;
+    int* d_counter = nullptr;
+    double* d_partial = nullptr;
+    int* d_small[3] = {nullptr, nullptr, nullptr};  // tier-1 lists per storage
+    int n_small[3] = {0, 0, 0};
+    int* d_large = nullptr;                         // every tier-2 solve
+    int n_large = 0;
+    int* d_init[3] = {nullptr, nullptr, nullptr};   // every solve per storage (initial residual)
+    int n_init[3] = {0, 0, 0};
+    std::vector<Tier2Class> classes;
+    std::vector<void*> owned;                       // per-set device allocations
+};
+
+static void free_set(ph3_batch* b) {
+    for (void* p : b->owned) (void)hipFree(p);
+    b->owned.clear();
+    b->classes.clear();
+    for (int s = 0; s < 3; ++s) { b->d_small[s] = b->d_init[s] = nullptr; b->n_small[s] = b->n_init[s] = 0; }
+    b->d_large = nullptr;
+    b->n_large = 0;
+}
+
+template <typename V>
+static int upload(ph3_batch* b, const std::vector<V>& v, V** out, hipStream_t s) {
+    *out = nullptr;
+    if (v.empty()) return PH3_OK;
+    void* p = nullptr;
+    PH3_CHECK(hipMalloc(&p, v.size() * sizeof(V)));
+    b->owned.push_back(p);
+    PH3_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, s));
+    *out = (V*)p;
+    return PH3_OK;
+}
+
+extern "C" size_t ph3_solve_desc_bytes(void) { return sizeof(ph3_solve_desc); }
+
+extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
+    if (!out || capacity < 1) { set_error("ph3_batch_create: bad arguments"); return PH3_EINVAL; }
+    ph3_batch* b = new ph3_batch();
+    b->cap = capacity;
+    hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(ph3_solve_desc) * (size_t)capacity);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, sizeof(int));
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&b->d_partial, sizeof(double) * ph3::kSlices * (size_t)capacity);
+    if (e != hipSuccess) {
+        (void)hipFree(b->d_desc); (void)hipFree(b->d_counter); (void)hipFree(b->d_partial);
+        delete b;
+        return ph3host::check(e, "ph3_batch_create");
+    }
+    *out = b;
+    return PH3_OK;
+}
+
+extern "C" int ph3_batch_destroy(ph3_batch* b) {
+    if (!b) return PH3_OK;
+    free_set(b);
+    (void)hipFree(b->d_desc);
+    (void)hipFree(b->d_counter);
+    (void)hipFree(b->d_partial);
+    delete b;
+    return PH3_OK;
+}
+
+static int check_desc(const ph3_solve_desc& d, int i) {
+    const bool shape_ok = (d.dims == 2 && d.nz == 1) || (d.dims == 3 && d.nz >= 3);
+    if (!shape_ok || d.ny < 3 || d.nx < 3) { set_error("solve %d: bad shape", i); return PH3_EINVAL; }
+    if (d.storage < 0 || d.storage > 2) { set_error("solve %d: bad storage", i); return PH3_EINVAL; }
+    if (!d.buf[0] || !d.buf[1] || d.buf[0] == d.buf[1]) { set_error("solve %d: buffers", i); return PH3_EINVAL; }
+    if (d.mode != 0 && d.mode != 1) { set_error("solve %d: mode %d", i, d.mode); return PH3_EINVAL; }
+    if (d.steps < 0 || d.stall_window < 0 || d.stall_window >= PH3_STALL_RING) {
+        set_error("solve %d: steps / stall_window out of range", i);
+        return PH3_EINVAL;
+    }
+    return PH3_OK;
+}
+
+extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, void* stream) {
+    if (!b || n < 0 || n > b->cap || (n && !descs)) { set_error("ph3_batch_set: bad arguments"); return PH3_EINVAL; }
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < n; ++i)
+        if (int r = check_desc(descs[i], i)) return r;
+    PH3_CHECK(hipStreamSynchronize(s));
+    free_set(b);
+    b->n = n;
+    if (n) PH3_CHECK(hipMemcpyAsync(b->d_desc, descs, sizeof(ph3_solve_desc) * n, hipMemcpyHostToDevice, s));
+    std::vector<int> small[3], init[3], large;
+    for (int i = 0; i < n; ++i) {
+        const ph3_solve_desc& d = descs[i];
+        init[d.storage].push_back(i);
+        const long long sites = (long long)d.nz * d.ny * d.nx;
+        if (sites <= PH3_BATCH_SMALL_SITES) { small[d.storage].push_back(i); continue; }
+        large.push_back(i);
+        Tier2Class* c = nullptr;
+        for (auto& k : b->classes)
+            if (k.storage == d.storage && k.dims == d.dims && k.periodic == (d.periodic != 0) &&
+                k.src == (d.src != nullptr)) c = &k;
+        if (!c) {
+            b->classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr, {}});
+            c = &b->classes.back();
+        }
+        c->idx.push_back(i);
+    }
+    for (int st = 0; st < 3; ++st) {
+        if (int r = upload(b, small[st], &b->d_small[st], s)) return r;
+        b->n_small[st] = (int)small[st].size();
+        if (int r = upload(b, init[st], &b->d_init[st], s)) return r;
+        b->n_init[st] = (int)init[st].size();
+    }
+    if (int r = upload(b, large, &b->d_large, s)) return r;
+    b->n_large = (int)large.size();
+    for (auto& c : b->classes) {
+        std::vector<long long> prefix(c.idx.size() + 1, 0);
+        for (size_t k = 0; k < c.idx.size(); ++k) {
+            const ph3_solve_desc& d = descs[c.idx[k]];
+            prefix[k + 1] = prefix[k] + ph3::tiles_of(d.dims, d.nz, d.ny, d.nx);
+        }
+        if (int r = upload(b, c.idx, &c.d_list, s)) return r;
+        if (int r = upload(b, prefix, &c.d_prefix, s)) return r;
+    }
+    // initial residuals of unchecked Jacobi solves
+    for (int st = 0; st < 3; ++st) {
+        if (!b->n_init[st]) continue;
+        const unsigned g = (unsigned)(b->n_init[st] < ph3::kPassGroups ? b->n_init[st] : ph3::kPassGroups);
+        switch (st) {
+            case ph3::kF32: ph3::batch_init_residual<float><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_init[st], b->n_init[st]); break;
+            case ph3::kF16: ph3::batch_init_residual<_Float16><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_init[st], b->n_init[st]); break;
+            default: ph3::batch_init_residual<ph3::bf16_t><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_init[st], b->n_init[st]); break;
+        }
+        PH3_LAUNCH_CHECK("batch initial residual");
+    }
+    PH3_CHECK(hipStreamSynchronize(s));
+    return PH3_OK;
+}
+
+template <typename T, bool P, bool S, int DIMS>
+static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
+    const unsigned g = (unsigned)ph3::kPassGroups;
+    ph3::batch_pass<T, P, S, DIMS><<<g, ph3::kWG, 0, s>>>(b->d_desc, c.d_list, c.d_prefix,
+                                                          (int)c.idx.size(), p);
+}
+
+template <typename T>
+static void pass_class(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
+    const bool P = c.periodic, S = c.src;
+    if (c.dims == 3) {
+        if (P) { if (S) launch_pass<T, true, true, 3>(c, b, p, s); else launch_pass<T, true, false, 3>(c, b, p, s); }
+        else { if (S) launch_pass<T, false, true, 3>(c, b, p, s); else launch_pass<T, false, false, 3>(c, b, p, s); }
+    } else {
+        if (P) { if (S) launch_pass<T, true, true, 2>(c, b, p, s); else launch_pass<T, true, false, 2>(c, b, p, s); }
+        else { if (S) launch_pass<T, false, true, 2>(c, b, p, s); else launch_pass<T, false, false, 2>(c, b, p, s); }
+    }
+}
+
+extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
+    if (!b || sweeps < 1) { set_error("ph3_batch_round: bad arguments"); return PH3_EINVAL; }
+    hipStream_t s = (hipStream_t)stream;
+    const int segs = (sweeps + PH3_BATCH_CHECK - 1) / PH3_BATCH_CHECK;
+    const int rs = segs * PH3_BATCH_CHECK;
+    // tier 1: one persistent launch per storage type present
+    for (int st = 0; st < 3; ++st) {
+        if (!b->n_small[st]) continue;
+        ph3::batch_reset<<<1, 1, 0, s>>>(b->d_counter);
+        PH3_LAUNCH_CHECK("batch counter reset");
+        const unsigned g = (unsigned)(b->n_small[st] < ph3::kPassGroups ? b->n_small[st] : ph3::kPassGroups);
+        switch (st) {
+            case ph3::kF32: ph3::batch_small<float><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
+            case ph3::kF16: ph3::batch_small<_Float16><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
+            default: ph3::batch_small<ph3::bf16_t><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
+        }
+        PH3_LAUNCH_CHECK("batch tier-1 launch");
+    }
+    // tier 2: segments of PH3_BATCH_CHECK varlen passes, then advance / residual / check
+    if (b->n_large) {
+        const unsigned gl = (unsigned)((b->n_large + 255) / 256);
+        const unsigned gr = (unsigned)(b->n_large * ph3::kSlices < ph3::kPassGroups ? b->n_large * ph3::kSlices : ph3::kPassGroups);
+        for (int seg = 0; seg < segs; ++seg) {
+            for (int p = 0; p < PH3_BATCH_CHECK; ++p) {
+                for (const auto& c : b->classes) {
+                    switch (c.storage) {
+                        case ph3::kF32: pass_class<float>(c, b, p, s); break;
+                        case ph3::kF16: pass_class<_Float16>(c, b, p, s); break;
+                        default: pass_class<ph3::bf16_t>(c, b, p, s); break;
+                    }
+                    PH3_LAUNCH_CHECK("batch tier-2 pass");
+                }
+            }
+            ph3::batch_advance<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->n_large);
+            PH3_LAUNCH_CHECK("batch advance");
+            bool st_used[3] = {false, false, false};
+            for (const auto& c : b->classes) st_used[c.storage] = true;
+            for (int st = 0; st < 3; ++st) {
+                if (!st_used[st]) continue;
+                switch (st) {
+                    case ph3::kF32: ph3::batch_residual<float><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial); break;
+                    case ph3::kF16: ph3::batch_residual<_Float16><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial); break;
+                    default: ph3::batch_residual<ph3::bf16_t><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial); break;
+                }
+                PH3_LAUNCH_CHECK("batch residual");
+            }
+            ph3::batch_check<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial);
+            PH3_LAUNCH_CHECK("batch check");
+        }
+    }
+    return PH3_OK;
+}
+
+extern "C" int ph3_batch_get(ph3_batch* b, ph3_solve_desc* descs, int n, void* stream) {
+    if (!b || n < 0 || n > b->n || (n && !descs)) { set_error("ph3_batch_get: bad arguments"); return PH3_EINVAL; }
+    if (n) PH3_CHECK(hipMemcpyAsync(descs, b->d_desc, sizeof(ph3_solve_desc) * n, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    PH3_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return PH3_OK;
+}

@@ -1,0 +1,302 @@
+// Tile bodies of the march rungs, shared by the single-grid kernels (sweep_zmarch.hip) and the
+// batched varlen kernels (batch.hip) so that both apply the same operations to every site.
+#pragma once
+#include "sweep_impl.h"
+#include "vec_io.h"
+
+namespace ph3 {
+
+template <int V>
+__device__ __forceinline__ void x_neighbours(const float (&c)[V], float hl, float hr, bool useL,
+                                             bool useR, float (&xm)[V], float (&xp)[V]) {
+    const float fromL = __shfl_up(c[V - 1], 1);
+    const float fromR = __shfl_down(c[0], 1);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        xm[e] = e > 0 ? c[e - 1] : (useL ? hl : fromL);
+        xp[e] = e < V - 1 ? c[e + 1] : (useR ? hr : fromR);
+    }
+}
+
+// Logical tile index of this workgroup: when the grid is a multiple of 8, workgroup L (which
+// the dispatcher places on XCD L % 8) takes logical tile (L % 8) * (G / 8) + L / 8, so every
+// XCD runs one contiguous range of logical tiles.
+__device__ __forceinline__ int logical_tile(int remap) {
+    const int L = blockIdx.x;
+    if (!remap) return L;
+    const int per = gridDim.x >> 3;
+    return (L & 7) * per + (L >> 3);
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_out(T* __restrict__ o, size_t at, const float (&w)[V],
+                                          bool nt) {
+    if constexpr (V * sizeof(T) == 16 && sizeof(T) == 4) {
+        if (nt) {
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            f32x4 v = {w[0], w[1], w[2], w[3]};
+            __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(o + at));
+            return;
+        }
+    }
+    store_vec<T, V>(o, at, w);
+}
+
+// One workgroup tile (tx, ty) of z chunk cz. Called by the single-grid kernel below and by the
+// batched varlen kernel, so a grid's sites get the same operations in both. Returns early only
+// uniformly over the workgroup. A caller that runs several tiles in a row must put a barrier
+// between them (the LDS slot is reused).
+template <typename T, bool P, bool S, int V, int R, int W>
+__device__ __forceinline__ void
+zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
+              int nz, int ny, int nx, float c0, float c1, int zch, int tx, int ty, int cz, int nt)
+{
+    __shared__ float edge[2][W][2][64 * V];  // [parity][wave][first/last row][site]
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int x = (tx * 64 + lane) * V;
+    const int J0 = ty * W * R;          // first row of the workgroup tile
+    const int j0 = J0 + wave * R;       // first row of this wave
+    int z0 = cz * zch, z1 = min(z0 + zch, nz);
+    if (!P) { z0 = max(z0, 1); z1 = min(z1, nz - 1); }
+    if (z0 >= z1) return;               // uniform over the workgroup
+
+    const bool xin = x < nx;
+    const int xl = xin ? x : nx - V;    // load column (in bounds for idle lanes)
+    const bool useL = lane == 0;
+    const bool useR = lane == 63 || x + V >= nx;
+    const int hxl = P ? (xl == 0 ? nx - 1 : xl - 1) : max(xl - 1, 0);
+    const int hxr = P ? (xl + V >= nx ? xl + V - nx : xl + V) : min(xl + V, nx - 1);
+    const size_t plane = (size_t)nx * ny;
+    const bool top = wave == 0, bot = wave == W - 1;
+
+    size_t roff[R];   // offset of row j0 + r within a plane (wrapped / clamped for loads)
+    bool rstore[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        const int jl = P ? j % ny : min(j, ny - 1);
+        roff[r] = (size_t)jl * nx;
+        rstore[r] = j < ny && (P || (j > 0 && j < ny - 1));
+    }
+    // rows just above / below the workgroup tile (read by the first / last wave)
+    const size_t hup = (size_t)(P ? (J0 + ny - 1) % ny : max(J0 - 1, 0)) * nx;
+    const size_t hdn = (size_t)(P ? (J0 + W * R) % ny : min(J0 + W * R, ny - 1)) * nx;
+
+    auto pidx = [&](int z) -> size_t {
+        const int zz = P ? (z < 0 ? z + nz : (z >= nz ? z - nz : z)) : z;
+        return (size_t)zz * plane;
+    };
+
+    float cm[R][V], cc[R][V], cn[R][V];
+    float yu[V], yd[V];      // memory halo rows of plane z (first / last wave only)
+    float xhl[R], xhr[R];    // x halos of plane z (used by edge lanes only)
+    float bs[R][V];          // source of plane z
+
+    {
+        const size_t pm = pidx(z0 - 1), p0 = pidx(z0), p1 = pidx(z0 + 1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            load_vec<T, V>(u, pm + roff[r] + xl, cm[r]);
+            load_vec<T, V>(u, p0 + roff[r] + xl, cc[r]);
+            load_vec<T, V>(u, p1 + roff[r] + xl, cn[r]);
+            xhl[r] = useL ? ld(u, p0 + roff[r] + hxl) : 0.f;
+            xhr[r] = useR ? ld(u, p0 + roff[r] + hxr) : 0.f;
+            if (S) load_vec<float, V>(b, p0 + roff[r] + xl, bs[r]);
+        }
+#pragma unroll
+        for (int e = 0; e < V; ++e) yu[e] = yd[e] = 0.f;
+        if (top) load_vec<T, V>(u, p0 + hup + xl, yu);
+        if (bot) load_vec<T, V>(u, p0 + hdn + xl, yd);
+    }
+
+    for (int z = z0; z < z1; ++z) {
+        // prefetch for plane z+1: its upper neighbour z+2, its halos and its source
+        float nn[R][V], nyu[V], nyd[V], nxl[R], nxr[R], nbs[R][V];
+        const bool more = z + 1 < z1;
+        if (more) {
+            const size_t p1 = pidx(z + 1), p2 = pidx(z + 2);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                load_vec<T, V>(u, p2 + roff[r] + xl, nn[r]);
+                nxl[r] = useL ? ld(u, p1 + roff[r] + hxl) : 0.f;
+                nxr[r] = useR ? ld(u, p1 + roff[r] + hxr) : 0.f;
+                if (S) load_vec<float, V>(b, p1 + roff[r] + xl, nbs[r]);
+            }
+            if (top) load_vec<T, V>(u, p1 + hup + xl, nyu);
+            if (bot) load_vec<T, V>(u, p1 + hdn + xl, nyd);
+        }
+
+        // exchange the first / last rows of plane z between the waves of the workgroup
+        const int par = (z - z0) & 1;
+        if (W > 1) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                edge[par][wave][0][lane * V + e] = cc[0][e];
+                edge[par][wave][1][lane * V + e] = cc[R - 1][e];
+            }
+            __syncthreads();
+            if (!top) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yu[e] = edge[par][wave - 1][1][lane * V + e];
+            }
+            if (!bot) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yd[e] = edge[par][wave + 1][0][lane * V + e];
+            }
+        }
+
+        const size_t pz = (size_t)z * plane;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float xm[V], xp[V];
+            x_neighbours<V>(cc[r], xhl[r], xhr[r], useL, useR, xm, xp);
+            float w[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const float ym = r > 0 ? cc[r - 1][e] : yu[e];
+                const float yp = r < R - 1 ? cc[r + 1][e] : yd[e];
+                float s = xm[e] + xp[e];
+                s = s + (ym + yp);
+                s = s + (cm[r][e] + cn[r][e]);
+                w[e] = update(c0, c1, cc[r][e], s);
+                if (S) w[e] = w[e] + bs[r][e];
+            }
+            if (xin && rstore[r]) {
+                const size_t at = pz + roff[r] + x;
+                if (P || (x > 0 && x + V < nx)) {
+                    store_out<T, V>(o, at, w, nt);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        if (x + e > 0 && x + e < nx - 1) st(o, at + e, w[e]);
+                }
+            }
+        }
+
+        if (more) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                xhl[r] = nxl[r];
+                xhr[r] = nxr[r];
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    cm[r][e] = cc[r][e];
+                    cc[r][e] = cn[r][e];
+                    cn[r][e] = nn[r][e];
+                    if (S) bs[r][e] = nbs[r][e];
+                }
+            }
+            if (top) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yu[e] = nyu[e];
+            }
+            if (bot) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) yd[e] = nyd[e];
+            }
+        }
+    }
+}
+
+// 2D: wave = 64*V columns; window of R+2 rows (y0-1 .. y0+R) in registers; R outputs per step.
+// Workgroup column group gx of row chunk cy; waves are independent (no LDS, no barrier).
+template <typename T, bool P, bool S, int V, int R, int W>
+__device__ __forceinline__ void
+ymarch2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
+              int ny, int nx, float c0, float c1, int ych, int gx, int cy)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int x = ((gx * W + wave) * 64 + lane) * V;
+    if ((gx * W + wave) * 64 * V >= nx) return;
+    int y0 = cy * ych, y1 = min(y0 + ych, ny);
+    if (!P) { y0 = max(y0, 1); y1 = min(y1, ny - 1); }
+    if (y0 >= y1) return;
+
+    const bool xin = x < nx;
+    const int xl = xin ? x : nx - V;
+    const bool useL = lane == 0;
+    const bool useR = lane == 63 || x + V >= nx;
+    const int hxl = P ? (xl == 0 ? nx - 1 : xl - 1) : max(xl - 1, 0);
+    const int hxr = P ? (xl + V >= nx ? xl + V - nx : xl + V) : min(xl + V, nx - 1);
+
+    auto ridx = [&](int y) -> size_t {
+        int yy = y;
+        if (P) { yy = y % ny; if (yy < 0) yy += ny; }
+        else yy = min(max(y, 0), ny - 1);
+        return (size_t)yy * nx;
+    };
+
+    // win[q] holds row y0 - 1 + q, q = 0 .. R+1
+    float win[R + 2][V];
+    float xhl[R], xhr[R], bs[R][V];
+#pragma unroll
+    for (int q = 0; q < R + 2; ++q) load_vec<T, V>(u, ridx(y0 - 1 + q) + xl, win[q]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const size_t ro = ridx(y0 + r);
+        xhl[r] = useL ? ld(u, ro + hxl) : 0.f;
+        xhr[r] = useR ? ld(u, ro + hxr) : 0.f;
+        if (S) load_vec<float, V>(b, ro + xl, bs[r]);
+    }
+
+    for (int y = y0; y < y1; y += R) {
+        // prefetch the next window's R new rows (y+R+1 .. y+2R), halos and source of y+R ..
+        float nw[R][V], nxl[R], nxr[R], nbs[R][V];
+        const bool more = y + R < y1;
+        if (more) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                load_vec<T, V>(u, ridx(y + R + 1 + r) + xl, nw[r]);
+                const size_t ro = ridx(y + R + r);
+                nxl[r] = useL ? ld(u, ro + hxl) : 0.f;
+                nxr[r] = useR ? ld(u, ro + hxr) : 0.f;
+                if (S) load_vec<float, V>(b, ro + xl, nbs[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int yr = y + r;
+            float xm[V], xp[V];
+            x_neighbours<V>(win[r + 1], xhl[r], xhr[r], useL, useR, xm, xp);
+            float w[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                float s = xm[e] + xp[e];
+                s = s + (win[r][e] + win[r + 2][e]);
+                w[e] = update(c0, c1, win[r + 1][e], s);
+                if (S) w[e] = w[e] + bs[r][e];
+            }
+            if (xin && yr < y1) {
+                const size_t at = (size_t)yr * nx + x;
+                if (P || (x > 0 && x + V < nx)) {
+                    store_vec<T, V>(o, at, w);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        if (x + e > 0 && x + e < nx - 1) st(o, at + e, w[e]);
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                win[0][e] = win[R][e];
+                win[1][e] = win[R + 1][e];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                xhl[r] = nxl[r];
+                xhr[r] = nxr[r];
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    win[r + 2][e] = nw[r][e];
+                    if (S) bs[r][e] = nbs[r][e];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace ph3

@@ -164,4 +164,33 @@ int ph3_event_elapsed_ms(float* ms, void* start, void* stop) {
     return PH3_OK;
 }
 
+int ph3_graph_begin(void* stream) {
+    if (!stream) { set_error("graph capture needs a created stream, not the null stream"); return PH3_EINVAL; }
+    PH3_CHECK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return PH3_OK;
+}
+
+int ph3_graph_end(void* stream, void** graph_exec) {
+    if (!graph_exec) { set_error("ph3_graph_end: graph_exec is NULL"); return PH3_EINVAL; }
+    *graph_exec = nullptr;
+    hipGraph_t g = nullptr;
+    PH3_CHECK(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess) return ph3host::check(r, "hipGraphInstantiate");
+    *graph_exec = (void*)e;
+    return PH3_OK;
+}
+
+int ph3_graph_launch(void* graph_exec, void* stream) {
+    PH3_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return PH3_OK;
+}
+
+int ph3_graph_destroy(void* graph_exec) {
+    if (graph_exec) PH3_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return PH3_OK;
+}
+
 }  // extern "C"

@@ -349,9 +349,12 @@ static int tb2_launch(const SweepArgs& a) {
 // 3D shapes (vec, rows per wave, waves): default (2, 4, 16); (1, 8, 8) for odd widths.
 template <typename T, bool P, bool S, int K>
 static int tb3_shape(const SweepArgs& a) {
-    int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), 2);
-    int r = a.tile.ty ? a.tile.ty : (v == 1 ? 8 : 4);
-    int w = a.tile.waves ? a.tile.waves : (v == 1 ? 8 : 16);
+    // Defaults measured on MI355X at 1024^3 float32 (vec, rows, waves): k=2 (4, 2, 16)
+    // 981 GLUPS; k=3 (2, 4, 16) 1291 GLUPS; k=4 (1, 4, 16) 977 GLUPS (register-bound).
+    const int vmax = K == 2 ? 4 : (K == 3 ? 2 : 1);
+    int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), vmax);
+    int r = a.tile.ty ? a.tile.ty : (v == 4 ? 2 : 4);
+    int w = a.tile.waves ? a.tile.waves : 16;
     if (a.nx % v) { ph3host::set_error("tb3d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
 #define PH3_TB3(V_, R_, W_) \
     if (v == V_ && r == R_ && w == W_) return tb3_launch<T, P, S, V_, R_, W_, K>(a)

@@ -44,6 +44,35 @@ __global__ void copy_tail_kernel(const uint8_t* __restrict__ src, uint8_t* __res
     if (i < n) dst[i] = src[i];
 }
 
+// Counter-based uniform generator: site i of stream `seed` is
+//   U = (splitmix64(seed * 0xD1B54A32D192ED03 + i) >> 40) * 2^-24, exactly representable.
+__host__ __device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// kind 0: U[0,1) inside, 0 on the outer layer; kind 1: U[0,1) on the outer layer, 0 inside;
+// kind 2: U[0,1) everywhere.
+template <typename T>
+__global__ void fill_random_kernel(T* __restrict__ dst, int dims, int nz, int ny, int nx,
+                                   uint64_t seed, int kind) {
+    const size_t n = (size_t)nz * ny * nx;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const uint64_t base = seed * 0xD1B54A32D192ED03ull;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int x = (int)(i % nx);
+        const size_t r = i / nx;
+        const int y = (int)(r % ny), z = (int)(r / ny);
+        const bool edge = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 ||
+                          (dims == 3 && (z == 0 || z == nz - 1));
+        float v = (float)(splitmix64(base + i) >> 40) * 5.9604644775390625e-08f;
+        if ((kind == 0 && edge) || (kind == 1 && !edge)) v = 0.f;
+        dst[i] = Codec<T>::store(v);
+    }
+}
+
 static unsigned grid_for(size_t n, unsigned block, unsigned cap) {
     size_t g = (n + block - 1) / block;
     if (g > cap) g = cap;
@@ -79,6 +108,26 @@ extern "C" int ph3_decode(int storage, const void* src, float* dst, size_t n, vo
         default: ph3host::set_error("decode: storage %d", storage); return PH3_EINVAL;
     }
     PH3_LAUNCH_CHECK("decode launch");
+    return PH3_OK;
+}
+
+extern "C" int ph3_fill_random(int storage, void* dst, int dims, int nz, int ny, int nx,
+                               unsigned long long seed, int kind, void* stream) {
+    if (dims == 2) nz = 1;
+    if (!dst || nx < 1 || ny < 1 || nz < 1 || kind < 0 || kind > 2) {
+        ph3host::set_error("fill_random: bad arguments");
+        return PH3_EINVAL;
+    }
+    const size_t n = (size_t)nz * ny * nx;
+    const unsigned g = grid_for(n, 256, 65536);
+    hipStream_t s = (hipStream_t)stream;
+    switch (storage) {
+        case kF32: fill_random_kernel<float><<<g, 256, 0, s>>>((float*)dst, dims, nz, ny, nx, seed, kind); break;
+        case kF16: fill_random_kernel<_Float16><<<g, 256, 0, s>>>((_Float16*)dst, dims, nz, ny, nx, seed, kind); break;
+        case kBF16: fill_random_kernel<bf16_t><<<g, 256, 0, s>>>((bf16_t*)dst, dims, nz, ny, nx, seed, kind); break;
+        default: ph3host::set_error("fill_random: storage %d", storage); return PH3_EINVAL;
+    }
+    PH3_LAUNCH_CHECK("fill_random launch");
     return PH3_OK;
 }
 

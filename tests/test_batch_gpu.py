@@ -1,0 +1,102 @@
+"""Batched solves on the GPU: parity with the oracle, alone == in the mix, batch == one by one.
+
+The mix spans both tiers (workgroup-owned grids of <= 32768 sites and varlen tile passes),
+2D and 3D, heat and Jacobi, float32 and bfloat16.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import numpy_ref as nr
+from pyheat3d_amd import _hip
+from pyheat3d_amd.analytic import splitmix_field
+from pyheat3d_amd.batch import BatchSolver, Request, solve_one_by_one
+
+pytestmark = pytest.mark.gpu
+
+
+def mix(storage="float32"):
+    shapes = [(20, 20), (33, 47), (130, 150), (200, 201), (12, 13, 14), (40, 36, 30),
+              (9, 70, 70), (5, 6, 7)]
+    reqs = []
+    for i, sh in enumerate(shapes):
+        for mode in ("heat", "jacobi"):
+            reqs.append(Request(shape=sh, mode=mode, storage=storage,
+                                steps=37 if mode == "heat" else 384, seed=1000 + 17 * i,
+                                rid=len(reqs)))
+    return reqs
+
+
+def run(reqs, graph=True, round_sweeps=128):
+    print(f"[gpu] batch of {len(reqs)} graph={graph}", flush=True)
+    b = BatchSolver(reqs, round_sweeps=round_sweeps, use_graph=graph)
+    try:
+        b.run()
+        return b.results(keep_fields=True)
+    finally:
+        b.close()
+
+
+def test_fill_random_matches_host_mirror():
+    stream = _hip.Stream()
+    for shape, kind in (((7, 9, 11), 0), ((13, 5), 1), ((3, 4, 5), 2)):
+        nz, ny, nx = (1, *shape) if len(shape) == 2 else shape
+        p = _hip.malloc(int(np.prod(shape)) * 4)
+        try:
+            _hip.check(_hip.lib().ph3_fill_random(0, p, len(shape), nz, ny, nx, 99, kind,
+                                                  stream.handle))
+            stream.sync()
+            got = np.empty(shape, np.float32)
+            _hip.memcpy(got.ctypes.data, p, got.nbytes, _hip.D2H)
+        finally:
+            _hip.free(p)
+        assert np.array_equal(got, splitmix_field(shape, 99, kind))
+    stream.close()
+
+
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_batch_matches_oracle(storage):
+    reqs = mix(storage)
+    res = run(reqs)
+    for req, r in zip(reqs, res):
+        u0 = splitmix_field(req.shape, req.seed, req.kind)
+        st = req.stencil()
+        assert r.status in ("done", "stagnated", "max_sweeps"), (req, r.status)
+        if req.mode == "heat":
+            assert r.status == "done" and r.steps == req.steps
+        ref = nr.emulated_reference(u0, st.c0, st.c1, "dirichlet", r.steps, storage, 1)
+        assert np.array_equal(r.u, ref), (req.shape, req.mode, r.steps, nr.rel_err(r.u, ref))
+        if req.mode == "jacobi":
+            # the relative residual reported by the device equals the oracle's FP64 residual
+            r0 = nr.residual_fp64(nr.round_storage(u0, storage), st.c1, "dirichlet")
+            rel = np.sqrt(nr.residual_fp64(ref, st.c1, "dirichlet") / r0)
+            assert r.rel_residual == pytest.approx(rel, rel=1e-9)
+            if r.status == "done":
+                assert r.rel_residual <= req.tol
+
+
+def test_alone_equals_in_mix():
+    reqs = mix()
+    together = run(reqs)
+    for req, r in zip(reqs, together):
+        alone = run([req])[0]
+        assert alone.status == r.status and alone.steps == r.steps, req
+        assert np.array_equal(alone.u, r.u), req
+
+
+def test_batch_equals_one_by_one_and_graph_equals_eager():
+    reqs = mix()
+    a = run(reqs, graph=True)
+    b = run(reqs, graph=False, round_sweeps=64)
+    c, _ = solve_one_by_one(reqs, keep_fields=True)
+    for req, x, y, z in zip(reqs, a, b, c):
+        assert x.steps == y.steps == z.steps, (req, x.steps, y.steps, z.steps)
+        assert np.array_equal(x.u, y.u) and np.array_equal(x.u, z.u), req
+
+
+def test_jacobi_converges_on_small_grid():
+    req = Request(shape=(17, 17), mode="jacobi", steps=100000, seed=5)
+    r = run([req])[0]
+    assert r.status == "done" and r.rel_residual <= 1e-6
+    predicted = np.log(1e-6) / np.log(np.cos(np.pi / 16))
+    assert r.steps < 3 * predicted
