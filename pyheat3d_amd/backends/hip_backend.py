@@ -53,7 +53,13 @@ class HipStepper(Stepper):
         except Exception:
             self.close()
             raise
-        self.tile = {"kind": self.kind, "k": self.steps_per_pass}
+        if self.tile_override is None and self.kind != "naive":
+            from ..tune import lookup
+
+            self.tile_override = lookup(self.kind, self.steps_per_pass, self.dims, self.storage,
+                                        self.bc, self.shape)
+        self.tile = {"kind": self.kind, "k": self.steps_per_pass,
+                     **(self.tile_override.as_dict() if self.tile_override else {})}
 
     _allow_2d = False
 
@@ -92,8 +98,10 @@ class HipStepper(Stepper):
 
     # -- stepping ----------------------------------------------------------------------------
     def _launch(self, kind: int, k: int) -> None:
+        # the tile override tunes the main kernel; remainder steps use the naive defaults
+        tile = self.tile_override if kind == self._kind_code else None
         _hip.sweep(kind, k, self.storage, self.periodic, self.shape, self._c0, self._c1,
-                   self.a, self.b, self.src or None, self.tile_override, self.stream.handle)
+                   self.a, self.b, self.src or None, tile, self.stream.handle)
         self.a, self.b = self.b, self.a
 
     def step(self, n: int = 1) -> None:

@@ -1,0 +1,225 @@
+"""Benchmark driver behind ``python -m pyheat3d_amd bench`` (reference ``src/pyheat3d/benchmark.py``).
+
+Single-grid timing keeps the reference's protocol and record schema (``benchmark.py:145-234``):
+warm-up of ``2k`` steps, calibration until a repeat lasts >= 0.2 ``min_time``, ``repeats`` timed
+repeats of ``n`` steps between device synchronisations, median MLUPS over interior sites,
+minimal-model bandwidth. Ceilings are MI355X ones: the copy bandwidth measured in the same
+process by this library's copy kernel, and the 8 TB/s spec.
+
+:func:`batch_benchmark` measures the batched server path against one launch per solve on the
+same request mix; :func:`serve_benchmark` runs the continuous-batching server on Poisson arrivals.
+"""
+
+from __future__ import annotations
+
+import datetime as _dt
+import json
+import math
+import os
+import platform
+import time
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+
+from . import traffic
+from .problem import heat_stencil
+from .registry import VARIANTS, availability, hip_available, make_stepper
+
+PLANS: dict[str, dict[str, Any]] = {
+    "quick": {"sizes": {2: [1024, 4096], 3: [64, 256]}, "min_time": 0.05, "repeats": 5,
+              "mem_limit_GB": 8.0, "out": "quick.json"},
+    "dev": {"sizes": {2: [512, 1024, 2048, 4096, 8192], 3: [32, 64, 128, 256, 320]},
+            "min_time": 0.1, "repeats": 5, "mem_limit_GB": 8.0, "out": "dev.json"},
+    "full": {"sizes": {2: [1024, 2048, 4096, 8192, 16384, 32768], 3: [64, 128, 256, 512, 1024]},
+             "min_time": 0.5, "repeats": 5, "mem_limit_GB": 64.0, "out": "mi355x.json"},
+}
+
+
+def machine_info() -> dict[str, Any]:
+    from . import _hip
+
+    info: dict[str, Any] = {"cpu": platform.processor() or None, "cpu_count": os.cpu_count(),
+                            "python": platform.python_version(), "numpy": np.__version__}
+    try:
+        d = _hip.device_info(0)
+        info.update(gpu=d["name"] or "MI355X", arch=d["arch"], cu_count=d["cu_count"],
+                    hbm_bytes=d["total_mem_bytes"])
+    except Exception as e:  # no device
+        info["gpu"] = f"unavailable: {e}"
+    info["date"] = _dt.datetime.now().astimezone().isoformat(timespec="seconds")
+    return info
+
+
+def time_variant(key: str, u0: np.ndarray, min_time: float, repeats: int,
+                 r: float | None = None, copy_bw: float | None = None,
+                 bc: str = "dirichlet") -> dict[str, Any]:
+    """Time one variant on one initial state; returns a reference-schema record."""
+    var = VARIANTS[key]
+    dims = u0.ndim
+    st = make_stepper(key, u0, heat_stencil(dims, r if r is not None else 0.9 / (2 * dims), bc))
+    k = var.steps_per_pass
+    try:
+        t0 = time.perf_counter()
+        st.step(2 * k)
+        st.synchronize()
+        warm = time.perf_counter() - t0
+        n = k
+        while True:
+            t0 = time.perf_counter()
+            st.step(n)
+            st.synchronize()
+            dt = time.perf_counter() - t0
+            if dt >= 0.2 * min_time or n >= 1 << 22:
+                break
+            n *= 2
+        n = max(k, int(math.ceil(min_time / (dt / n) / k)) * k)
+        times = []
+        for _ in range(repeats):
+            st.synchronize()
+            t0 = time.perf_counter()
+            st.step(n)
+            st.synchronize()
+            times.append(time.perf_counter() - t0)
+        finite = bool(np.isfinite(st.get()).all())
+    finally:
+        st.close()
+    ncells = math.prod(s - 2 for s in u0.shape) if bc == "dirichlet" else math.prod(u0.shape)
+    mlups = [ncells * n / t / 1e6 for t in times]
+    med = float(np.median(mlups))
+    bpu = traffic.minimal_bytes(var.bytes_per_value, k)
+    eff = med * bpu / 1e3
+    rec: dict[str, Any] = {
+        "backend": var.backend, "variant": var.name, "key": key, "rung": var.rung, "dims": dims,
+        "size": list(u0.shape), "bc": bc, "dtype": var.storage, "metric": "MLUPS", "value": med,
+        "repeats": repeats, "median": med, "min": float(min(mlups)), "max": float(max(mlups)),
+        "steps_per_repeat": n, "seconds": [float(t) for t in times], "steps_per_pass": k,
+        "bytes_per_unit": bpu, "effective_GBs": eff,
+        "pct_copy_bw": 100.0 * eff / copy_bw if copy_bw else None,
+        "pct_spec_bw": 100.0 * eff / traffic.SPEC_BW_GBS,
+        "roofline_MLUPS": traffic.roofline_mlups(copy_bw or traffic.SPEC_BW_GBS, bpu),
+        "flops_per_unit": traffic.flops_per_update(dims), "GFLOPs": med * traffic.flops_per_update(dims) / 1e3,
+        "warmup_s": warm, "finite": finite, "notes": var.description,
+    }
+    if k > 1:
+        rec["single_step_equiv_GBs"] = med * traffic.minimal_bytes(var.bytes_per_value, 1) / 1e3
+    return rec
+
+
+def run(mode: str = "quick", out_dir: str | Path = "results", variants: list[str] | None = None,
+        dims_list: tuple[int, ...] = (2, 3), out_name: str | None = None,
+        verbose: bool = True) -> dict[str, Any]:
+    """Run a plan over sizes x variants and write the results JSON (reference ``run``)."""
+    from .roofline import copy_bandwidth
+
+    plan = PLANS[mode]
+    out_dir = Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    out_path = out_dir / (out_name or plan["out"])
+    keys = [k for k in VARIANTS if variants is None or any(v in k for v in variants)]
+    keys = [k for k in keys if availability(k)[0]]
+    copy_bw = copy_bandwidth(1 << 30) if hip_available() else None
+    result: dict[str, Any] = {
+        "machine": machine_info(), "mode": mode,
+        "ceiling": {"measured_copy_bandwidth_GBs": copy_bw, "spec_bandwidth_GBs": traffic.SPEC_BW_GBS},
+        "traffic_model": "minimal: 2*bytes_per_value/steps_per_pass per lattice-site update",
+        "runs": [],
+    }
+    t_start = time.perf_counter()
+    rng = np.random.default_rng(12345)
+    for dims in dims_list:
+        for n in plan["sizes"][dims]:
+            shape = (n,) * dims
+            u0 = rng.random(shape, dtype=np.float32)
+            for key in keys:
+                var = VARIANTS[key]
+                if dims not in var.dims or 2 * math.prod(shape) * var.bytes_per_value / 1e9 > plan["mem_limit_GB"]:
+                    continue
+                try:
+                    rec = time_variant(key, u0, plan["min_time"], plan["repeats"], copy_bw=copy_bw)
+                except Exception as e:  # keep sweeping, record the failure
+                    print(f"FAILED {key} {shape}: {type(e).__name__}: {e}")
+                    continue
+                result["runs"].append(rec)
+                if verbose:
+                    print(f"{dims}D {n:>6} {key:26s} {rec['median'] / 1e3:8.2f} GLUPS "
+                          f"{rec['effective_GBs']:8.1f} GB/s", flush=True)
+            out_path.write_text(json.dumps(result, indent=1))
+    result["wall_time_s"] = time.perf_counter() - t_start
+    out_path.write_text(json.dumps(result, indent=1))
+    return result
+
+
+# ----------------------------------------------------------------------------------------------
+# batched and served solves
+# ----------------------------------------------------------------------------------------------
+
+def _lat_stats(ms: np.ndarray) -> dict[str, float]:
+    return {"p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
+            "max_ms": float(ms.max())}
+
+
+def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 512),
+                    n3: tuple[int, int] = (32, 128), frac_jacobi: float = 0.5,
+                    storage: str = "float32", baseline_every: int = 8, round_sweeps: int = 256,
+                    jacobi_cap: int = 10 ** 6) -> dict[str, Any]:
+    """All requests submitted at t = 0: batched graph rounds vs one launch per sweep per solve.
+
+    The per-solve baseline runs every ``baseline_every``-th request (a systematic sample of the
+    same mix); its solves/s is the sample size over the sum of its solve times.
+    """
+    from .batch import BatchSolver, make_mix, solve_one_by_one
+
+    reqs = make_mix(n, seed, n2=n2, n3=n3, frac_jacobi=frac_jacobi, storage=storage,
+                    jacobi_cap=jacobi_cap)
+    bs = BatchSolver(reqs, round_sweeps=round_sweeps)
+    try:
+        timing = bs.run()
+        res = bs.results()
+    finally:
+        bs.close()
+    sample = reqs[::baseline_every]
+    base, _ = solve_one_by_one(sample)
+    base_time = sum(r.seconds for r in base)
+    jac = [r for q, r in zip(reqs, res) if q.mode == "jacobi"]
+    by_rid = {r.rid: r for r in res}
+    agree = sum(1 for b in base if by_rid[b.rid].steps == b.steps and by_rid[b.rid].status == b.status)
+    return {
+        "workload": f"{n} solves, {int(100 * (1 - frac_jacobi))}% heat (200 steps) / "
+                    f"{int(100 * frac_jacobi)}% Jacobi Laplace to 1e-6, 60% 2D N in {list(n2)}, "
+                    f"40% 3D N in {list(n3)}, {storage}, seed {seed}",
+        "batched": {"seconds": timing["seconds"], "rounds": timing["rounds"],
+                    "solves_per_s": n / timing["seconds"]},
+        "per_solve": {"sample": len(sample), "seconds": base_time,
+                      "solves_per_s": len(sample) / base_time,
+                      **_lat_stats(np.array([r.seconds for r in base]) * 1e3)},
+        "speedup": (n / timing["seconds"]) / (len(sample) / base_time),
+        "jacobi": {"converged": sum(r.status == "done" for r in jac),
+                   "stagnated": sum(r.status == "stagnated" for r in jac),
+                   "capped": sum(r.status == "max_sweeps" for r in jac),
+                   "max_rel_residual": max((r.rel_residual or 0.0) for r in jac) if jac else None},
+        "baseline_agrees_on_steps_and_status": f"{agree}/{len(sample)}",
+    }
+
+
+def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
+                    n2: tuple[int, int] = (64, 512), n3: tuple[int, int] = (32, 128),
+                    storage: str = "float32", budget_gb: float = 200.0,
+                    round_sweeps: int = 128) -> dict[str, Any]:
+    from .batch import make_mix
+    from .server import Server, poisson_arrivals
+
+    reqs = make_mix(n, seed, n2=n2, n3=n3, storage=storage)
+    arr = poisson_arrivals(n, rate, seed + 1)
+    srv = Server(int(budget_gb * 1e9), round_sweeps=round_sweeps)
+    try:
+        res, st = srv.serve(reqs, arr)
+    finally:
+        srv.close()
+    return {"workload": f"{n} Poisson arrivals at {rate:g}/s, mix as the batch benchmark, "
+                        f"{storage}, HBM budget {budget_gb:g} GB",
+            "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+            "max_ms": st.max_ms, "seconds": st.seconds, "rounds": st.rounds,
+            "max_resident": st.max_resident, "max_resident_GB": st.max_resident_bytes / 1e9,
+            "statuses": {s: sum(r.status == s for r in res) for s in ("done", "stagnated", "max_sweeps")}}

@@ -1,0 +1,210 @@
+"""Command line: ``python -m pyheat3d_amd {list, run, validate, bench, batch, serve, jacobi, tune}``.
+
+``list``, ``run``, ``validate`` and ``bench`` mirror the reference CLI (``src/pyheat3d/cli.py``);
+``batch``, ``serve``, ``jacobi`` and ``tune`` drive the batched solver, the serving loop, the
+Jacobi-to-tolerance solve of the manufactured Poisson problem and the autotuner.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import analytic as an
+from .problem import heat_stencil, jacobi_stencil, stability_limit
+from .registry import VARIANTS, availability, make_stepper
+
+
+def _cmd_list(_: argparse.Namespace) -> int:
+    print(f"{'variant':30s} {'dims':6s} {'storage':9s} {'k':>2s}  status")
+    for key, v in VARIANTS.items():
+        ok, why = availability(key)
+        dims = ",".join(str(d) for d in v.dims)
+        print(f"{key:30s} {dims:6s} {v.storage:9s} {v.steps_per_pass:>2d}  "
+              f"{'ok' if ok else 'unavailable (' + why + ')'}")
+    return 0
+
+
+def _cmd_run(a: argparse.Namespace) -> int:
+    dims = a.dims
+    shape = (a.n,) * dims
+    if a.mode == "heat":
+        r = a.r if a.r is not None else 0.9 * stability_limit(dims)
+        st = heat_stencil(dims, r, a.bc)
+        u0 = (an.fourier_mode(a.n, dims, a.bc) if a.init == "mode"
+              else np.random.default_rng(0).random(shape)).astype(np.float32)
+    else:
+        st = jacobi_stencil(dims, a.bc)
+        u0 = an.harmonic_sinh(a.n, dims).astype(np.float32)
+        u0[(slice(1, -1),) * dims] = 0.0
+    s = make_stepper(a.variant, u0, st)
+    try:
+        s.step(s.steps_per_pass)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(a.steps)
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        u = s.get()
+    finally:
+        s.close()
+    print(f"{a.variant}: {dims}D n={a.n} {a.mode} {a.bc}, {a.steps} steps in {dt:.3f} s -> "
+          f"{s.ncells * a.steps / dt / 1e9:.2f} GLUPS (development timing, not a benchmark)")
+    print(f"state: min {u.min():.4g} max {u.max():.4g} mean {u.mean():.4g}")
+    return 0
+
+
+def _cmd_validate(a: argparse.Namespace) -> int:
+    from .validation import run_all
+
+    res = run_all(quick=a.quick)
+    bad = 0
+    for x in res["spatial"]:
+        ok = 1.85 < x["order"] < 2.15
+        bad += not ok
+        print(f"spatial   {x['variant']:22s} {x['dims']}D order {x['order']:.3f} {'ok' if ok else 'FAIL'}")
+    for x in res["temporal"]:
+        ok = 0.95 < x["order"] < 1.05
+        bad += not ok
+        print(f"temporal  {x['variant']:22s} {x['dims']}D order {x['order']:.3f} {'ok' if ok else 'FAIL'}")
+    for x in res["laplace_quadratic"]:
+        print(f"quadratic {x['variant']:22s} {x['dims']}D n={x['n']} {x['status']} after "
+              f"{x['sweeps']} sweeps, err {x['err']:.2e}")
+    for x in res["poisson_eigen"]:
+        print(f"poisson   {x['variant']:22s} n={x['n']} {x['status']} sweeps {x['sweeps']} "
+              f"(predicted {x['predicted_sweeps']}), err vs FP64 {x['err_vs_fp64']:.2e}")
+    if a.out:
+        Path(a.out).parent.mkdir(parents=True, exist_ok=True)
+        Path(a.out).write_text(json.dumps(res, indent=1))
+    return 1 if bad else 0
+
+
+def _cmd_bench(a: argparse.Namespace) -> int:
+    from .benchmark import run
+
+    run("full" if a.full else ("dev" if a.dev else "quick"), a.out_dir, a.variants, tuple(a.dims))
+    return 0
+
+
+def _cmd_batch(a: argparse.Namespace) -> int:
+    from .benchmark import batch_benchmark
+
+    r = batch_benchmark(a.n, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
+                        a.frac_jacobi, a.storage, a.baseline_every, a.round_sweeps)
+    print(json.dumps(r, indent=1))
+    return 0
+
+
+def _cmd_serve(a: argparse.Namespace) -> int:
+    from .benchmark import serve_benchmark
+
+    r = serve_benchmark(a.n, a.rate, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
+                        a.storage, a.budget_gb, a.round_sweeps)
+    print(json.dumps(r, indent=1))
+    return 0
+
+
+def _cmd_jacobi(a: argparse.Namespace) -> int:
+    from .validation import poisson_eigen
+
+    r = poisson_eigen(a.variant, a.n, 3, a.tol, a.check_every, a.max_sweeps)
+    print(json.dumps(r, indent=1))
+    return 0
+
+
+def _cmd_tune(a: argparse.Namespace) -> int:
+    from .tune import tune, update_table
+
+    entries = []
+    for key in a.variants:
+        dims = a.dims if a.dims in VARIANTS[key].dims else VARIANTS[key].dims[0]
+        e = tune(key, (a.n,) * dims, a.bc)
+        print(f"best {key}: {e['tile']} {e['glups']:.1f} GLUPS")
+        entries.append(e)
+    if a.write:
+        update_table(entries)
+    return 0
+
+
+def _size_args(p: argparse.ArgumentParser) -> None:
+    p.add_argument("--n2-min", type=int, default=64)
+    p.add_argument("--n2-max", type=int, default=512)
+    p.add_argument("--n3-min", type=int, default=32)
+    p.add_argument("--n3-max", type=int, default=128)
+    p.add_argument("--storage", default="float32", choices=("float32", "float16", "bfloat16"))
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--round-sweeps", type=int, default=256)
+
+
+def main(argv: list[str] | None = None) -> int:
+    p = argparse.ArgumentParser(prog="pyheat3d_amd", description=__doc__)
+    sub = p.add_subparsers(dest="cmd", required=True)
+    sub.add_parser("list", help="list variants and availability").set_defaults(fn=_cmd_list)
+
+    r = sub.add_parser("run", help="run one variant once and print a rough rate")
+    r.add_argument("--variant", default="hip-cdna4/naive", choices=sorted(VARIANTS))
+    r.add_argument("--dims", type=int, default=3, choices=(2, 3))
+    r.add_argument("--n", type=int, default=64)
+    r.add_argument("--steps", type=int, default=100)
+    r.add_argument("--bc", default="dirichlet", choices=("dirichlet", "periodic"))
+    r.add_argument("--mode", default="heat", choices=("heat", "jacobi"))
+    r.add_argument("--r", type=float, default=None)
+    r.add_argument("--init", default="mode", choices=("mode", "random"))
+    r.set_defaults(fn=_cmd_run)
+
+    v = sub.add_parser("validate", help="convergence studies against closed forms")
+    v.add_argument("--quick", action="store_true")
+    v.add_argument("--out", default=None)
+    v.set_defaults(fn=_cmd_validate)
+
+    b = sub.add_parser("bench", help="single-grid benchmark sweep (as bench/run.py)")
+    g = b.add_mutually_exclusive_group()
+    g.add_argument("--quick", action="store_true")
+    g.add_argument("--full", action="store_true")
+    g.add_argument("--dev", action="store_true")
+    b.add_argument("--variants", nargs="*", default=None)
+    b.add_argument("--dims", type=int, nargs="*", default=[2, 3])
+    b.add_argument("--out-dir", default="results")
+    b.set_defaults(fn=_cmd_bench)
+
+    ba = sub.add_parser("batch", help="batched solves vs one launch per solve")
+    ba.add_argument("--n", type=int, default=2048)
+    ba.add_argument("--frac-jacobi", type=float, default=0.5)
+    ba.add_argument("--baseline-every", type=int, default=8)
+    _size_args(ba)
+    ba.set_defaults(fn=_cmd_batch)
+
+    se = sub.add_parser("serve", help="continuous-batching server on Poisson arrivals")
+    se.add_argument("--n", type=int, default=1024)
+    se.add_argument("--rate", type=float, default=500.0)
+    se.add_argument("--budget-gb", type=float, default=200.0)
+    _size_args(se)
+    se.set_defaults(fn=_cmd_serve)
+
+    j = sub.add_parser("jacobi", help="Jacobi to tolerance on the manufactured Poisson problem")
+    j.add_argument("--variant", default="hip-cdna4/zmarch", choices=sorted(VARIANTS))
+    j.add_argument("--n", type=int, default=768)
+    j.add_argument("--tol", type=float, default=1e-6)
+    j.add_argument("--check-every", type=int, default=500)
+    j.add_argument("--max-sweeps", type=int, default=3 * 10 ** 6)
+    j.set_defaults(fn=_cmd_jacobi)
+
+    t = sub.add_parser("tune", help="autotune launch shapes and update the tuning table")
+    t.add_argument("--variants", nargs="+", default=["hip-cdna4/zmarch", "hip-cdna4/tb3"])
+    t.add_argument("--dims", type=int, default=3)
+    t.add_argument("--n", type=int, default=1024)
+    t.add_argument("--bc", default="periodic", choices=("dirichlet", "periodic"))
+    t.add_argument("--write", action="store_true")
+    t.set_defaults(fn=_cmd_tune)
+
+    a = p.parse_args(argv)
+    return a.fn(a)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

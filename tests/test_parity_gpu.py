@@ -153,6 +153,59 @@ def test_zmarch_every_tile_shape(storage):
                     assert np.array_equal(u, ref), (r, w, vec, zch, nr.rel_err(u, ref))
 
 
+TB3_SHAPES = [(2, 4, 16), (1, 8, 8), (2, 4, 8), (4, 2, 16), (1, 4, 16)]  # (vec, rows, waves)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_tb3d_every_tile_shape(k):
+    """Every built 3D temporal-blocking shape and lane vector (even widths), both BCs, source."""
+    rng = np.random.default_rng(16 + k)
+    for shape, bc in (((21, 67, 136), "periodic"), ((17, 40, 72), "dirichlet")):
+        u0 = rng.random(shape, dtype=np.float32)
+        b = rng.random(shape, dtype=np.float32) * np.float32(1e-3)
+        st = heat_stencil(3, 0.15, bc)
+        for src in (None, b):
+            ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 2 * k, b=src)
+            for (vec, r, w) in TB3_SHAPES:
+                for zch in (0, 5):
+                    s = make_stepper(f"hip-cdna4/tb{k}", u0, st, src)
+                    s.tile_override = _hip.Tile(r, zch, vec, w, 0)
+                    print(f"[gpu] tb{k} {shape} {bc} src={src is not None} vec={vec} rows={r} "
+                          f"waves={w} zch={zch}", flush=True)
+                    try:
+                        s.step(2 * k)
+                        u = s.get()
+                    finally:
+                        s.close()
+                    assert np.array_equal(u, ref), (k, bc, vec, r, w, zch, nr.rel_err(u, ref))
+
+
+@pytest.mark.parametrize("key", ["hip-cdna4/tb2", "hip-cdna4/tb3", "hip-cdna4/tb4",
+                                 "hip-cdna4/tb8", "hip-cdna4/tb4-bf16"])
+def test_tb2d_lane_vectors(key):
+    """2D temporal blocking with every lane vector width (even and odd widths), both BCs."""
+    rng = np.random.default_rng(30)
+    k = VARIANTS[key].steps_per_pass
+    storage = VARIANTS[key].storage
+    for shape in ((37, 136), (50, 264), (19, 97)):
+        u0 = rng.random(shape, dtype=np.float32)
+        for bc in ("dirichlet", "periodic"):
+            st = heat_stencil(2, 0.2, bc)
+            ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 2 * k + 1, storage, k)
+            for vec in (1, 2, 4, 8):
+                if shape[1] % vec or (vec == 8 and storage == "float32"):
+                    continue
+                s = make_stepper(key, u0, st)
+                s.tile_override = _hip.Tile(0, 0, vec, 4, 0)
+                print(f"[gpu] {key} {shape} {bc} vec={vec}", flush=True)
+                try:
+                    s.step(2 * k + 1)
+                    u = s.get()
+                finally:
+                    s.close()
+                assert np.array_equal(u, ref), (key, shape, bc, vec, nr.rel_err(u, ref))
+
+
 def test_residual_matches_oracle():
     rng = np.random.default_rng(21)
     for shape in ((33, 40, 47), (61, 75)):
