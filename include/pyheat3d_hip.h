@@ -58,7 +58,9 @@ int ph3_malloc(void** ptr, size_t bytes);
 int ph3_free(void* ptr);
 int ph3_host_alloc(void** ptr, size_t bytes);      /* pinned host memory */
 int ph3_host_free(void* ptr);
-/* kind: 0 host->device, 1 device->host, 2 device->device. Synchronous. */
+/* kind: 0 host->device, 1 device->host, 2 device->device. ph3_memcpy returns after the copy
+ * has completed on the device (safe to follow with launches on any stream); the _async form is
+ * ordered on `stream` only. */
 int ph3_memcpy(void* dst, const void* src, size_t bytes, int kind);
 int ph3_memcpy_async(void* dst, const void* src, size_t bytes, int kind, void* stream);
 

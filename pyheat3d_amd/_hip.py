@@ -176,7 +176,13 @@ def free(ptr: int) -> None:
 
 
 def memcpy(dst: int, src: int, nbytes: int, kind: int) -> None:
+    """Blocking copy, complete on the device when it returns."""
     check(lib().ph3_memcpy(dst, src, nbytes, kind), "memcpy")
+
+
+def memcpy_async(dst: int, src: int, nbytes: int, kind: int, stream: "Stream") -> None:
+    """Copy ordered on ``stream`` (host buffers must stay alive until the stream is synced)."""
+    check(lib().ph3_memcpy_async(dst, src, nbytes, kind, stream.handle), "memcpy_async")
 
 
 def mem_info() -> tuple[int, int]:

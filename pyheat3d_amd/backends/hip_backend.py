@@ -65,20 +65,23 @@ class HipStepper(Stepper):
 
     # -- host <-> device ---------------------------------------------------------------------
     def _upload(self, host: np.ndarray) -> None:
+        # Every copy is ordered on this stepper's stream and completed before the first sweep:
+        # the stream is non-blocking, so it does not wait for work on the null stream.
         code = _hip.STORAGE_CODE[self.storage]
+        s = self.stream
         if self.storage == "float32":
-            _hip.memcpy(self.a, host.ctypes.data, host.nbytes, _hip.H2D)
+            _hip.memcpy_async(self.a, host.ctypes.data, host.nbytes, _hip.H2D, s)
         else:
             tmp = _hip.malloc(host.nbytes)
             try:
-                _hip.memcpy(tmp, host.ctypes.data, host.nbytes, _hip.H2D)
-                _hip.check(_hip.lib().ph3_encode(code, tmp, self.a, self.n, self.stream.handle),
-                           "encode")
-                self.stream.sync()
+                _hip.memcpy_async(tmp, host.ctypes.data, host.nbytes, _hip.H2D, s)
+                _hip.check(_hip.lib().ph3_encode(code, tmp, self.a, self.n, s.handle), "encode")
+                s.sync()
             finally:
                 _hip.free(tmp)
         # both ping-pong buffers carry the (fixed) Dirichlet boundary
-        _hip.memcpy(self.b, self.a, self.nbytes, _hip.D2D)
+        _hip.memcpy_async(self.b, self.a, self.nbytes, _hip.D2D, s)
+        s.sync()
 
     def get(self) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.float32)

@@ -106,10 +106,16 @@ static int copy_kind(int kind, hipMemcpyKind* k) {
     return PH3_EINVAL;
 }
 
+// Complete before returning: hipMemcpy alone may return before a device-to-device copy (or the
+// DMA of a pageable host-to-device copy) has landed, and kernels on non-blocking streams are not
+// ordered after the null stream, so the copy is followed by a wait on the stream it ran on.
 int ph3_memcpy(void* dst, const void* src, size_t bytes, int kind) {
     hipMemcpyKind k;
     if (int r = copy_kind(kind, &k)) return r;
-    if (bytes) PH3_CHECK(hipMemcpy(dst, src, bytes, k));
+    if (bytes) {
+        PH3_CHECK(hipMemcpyAsync(dst, src, bytes, k, nullptr));
+        PH3_CHECK(hipStreamSynchronize(nullptr));
+    }
     return PH3_OK;
 }
 
