@@ -54,7 +54,9 @@ def time_variant(key: str, u0: np.ndarray, stencil, steps: int, warmup: int,
     from pyheat3d_amd.registry import VARIANTS, make_stepper
 
     var = VARIANTS[key]
-    st = make_stepper(key, u0, stencil)
+    t_up = time.perf_counter()
+    st = make_stepper(key, u0, stencil)  # allocation + host-to-device upload of u0
+    t_up = time.perf_counter() - t_up
     try:
         k = var.steps_per_pass
         steps = max(k, steps - steps % k)
@@ -70,7 +72,9 @@ def time_variant(key: str, u0: np.ndarray, stencil, steps: int, warmup: int,
         ev_ms = ev0.elapsed_ms(ev1)
         ev0.close()
         ev1.close()
-        out = st.get()
+        t_down = time.perf_counter()
+        out = st.get()  # device-to-host download of the final field
+        t_down = time.perf_counter() - t_down
         total_steps = st.steps_done
         finite = bool(np.isfinite(out).all())
         check = None
@@ -96,7 +100,10 @@ def time_variant(key: str, u0: np.ndarray, stencil, steps: int, warmup: int,
             "kernel_avg_ms": ev_ms / launches, "bytes_per_update": bpu,
             "algorithmic_bytes_per_launch": bpu * k * cells,
             "effective_GBs": cells * steps / wall * bpu / 1e9, "finite": finite,
-            "check": check}
+            "check": check,
+            "host_transfer": {"upload_s": round(t_up, 4), "download_s": round(t_down, 4),
+                              "bytes_each_way": int(u0.nbytes),
+                              "glups_including_transfers": cells * steps / (wall + t_up + t_down) / 1e9}}
 
 
 def load_traffic(workload: str) -> dict | None:
@@ -215,6 +222,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "check": {"finite": head["finite"], **head["check"],
                   "what": "amplitude of each sine mode after the run vs the closed form g_m^steps"},
+        "host_transfer": head["host_transfer"],
         "rungs": others,
         "device": {"name": info["name"], "arch": info["arch"], "cu_count": info["cu_count"]},
     }

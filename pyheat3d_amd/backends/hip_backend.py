@@ -118,24 +118,24 @@ class HipStepper(Stepper):
     def synchronize(self) -> None:
         self.stream.sync()
 
+    _work = 0
+
     def residual_sq(self) -> float:
         """FP64 squared 2-norm of ``b + c0 u + c1 S(u) - u`` for the current state (blocking)."""
-        work = _hip.malloc(int(_hip.lib().ph3_residual_work_bytes()) + 256)
-        try:
-            out = work  # first 8 bytes hold the result, partials follow at +256
-            _hip.check(_hip.lib().ph3_residual(
-                _hip.STORAGE_CODE[self.storage], int(self.periodic), self.dims,
-                *((1, *self.shape) if self.dims == 2 else self.shape), self._c0, self._c1,
-                self.a, self.src or None, out, work + 256, self.stream.handle), "residual")
-            self.stream.sync()
-            res = ctypes.c_double()
-            _hip.memcpy(ctypes.addressof(res), out, 8, _hip.D2H)
-        finally:
-            _hip.free(work)
+        if not self._work:  # result at offset 0, workgroup partials from offset 256
+            self._work = _hip.malloc(int(_hip.lib().ph3_residual_work_bytes()) + 256)
+        _hip.check(_hip.lib().ph3_residual(
+            _hip.STORAGE_CODE[self.storage], int(self.periodic), self.dims,
+            *((1, *self.shape) if self.dims == 2 else self.shape), self._c0, self._c1,
+            self.a, self.src or None, self._work, self._work + 256, self.stream.handle),
+            "residual")
+        self.stream.sync()
+        res = ctypes.c_double()
+        _hip.memcpy(ctypes.addressof(res), self._work, 8, _hip.D2H)
         return float(res.value)
 
     def close(self) -> None:
-        for name in ("a", "b", "src"):
+        for name in ("a", "b", "src", "_work"):
             p = getattr(self, name, 0)
             if p:
                 _hip.free(p)

@@ -112,7 +112,15 @@ def _cmd_serve(a: argparse.Namespace) -> int:
 def _cmd_jacobi(a: argparse.Namespace) -> int:
     from .validation import poisson_eigen
 
-    r = poisson_eigen(a.variant, a.n, 3, a.tol, a.check_every, a.max_sweeps)
+    last = [time.perf_counter()]
+
+    def progress(sweeps: int, rel: float) -> None:
+        if time.perf_counter() - last[0] > 10.0:
+            last[0] = time.perf_counter()
+            print(f"{a.variant} n={a.n}: {sweeps} sweeps, rel residual {rel:.3e}",
+                  file=sys.stderr, flush=True)
+
+    r = poisson_eigen(a.variant, a.n, 3, a.tol, a.check_every, a.max_sweeps, progress)
     print(json.dumps(r, indent=1))
     return 0
 

@@ -51,7 +51,7 @@ class JacobiResult:
 def solve(key: str, u0: np.ndarray, source: np.ndarray | None = None, bc: str = "dirichlet",
           tol: float = 1e-6, check_every: int = 100, max_sweeps: int = 10 ** 7,
           stall_window: int = 20, stall_fraction: float = 0.05, keep: bool = True,
-          stepper=None) -> JacobiResult:
+          stepper=None, progress=None) -> JacobiResult:
     """Jacobi sweeps of variant ``key`` from ``u0`` until converged, stagnated or capped."""
     st = stepper or make_stepper(key, u0, jacobi_stencil(u0.ndim, bc), source)
     k = st.steps_per_pass
@@ -71,6 +71,8 @@ def solve(key: str, u0: np.ndarray, source: np.ndarray | None = None, bc: str = 
             sweeps += n
             rel = math.sqrt(st.residual_sq() / r0)
             hist.append((sweeps, rel))
+            if progress is not None:
+                progress(sweeps, rel)
             if rel <= tol:
                 status = "converged"
                 break

@@ -83,19 +83,30 @@ def laplace_quadratic(key: str, dims: int, n: int, tol: float = 1e-6) -> dict[st
 
 
 def poisson_eigen(key: str, n: int, dims: int = 3, tol: float = 1e-6,
-                  check_every: int = 100, max_sweeps: int = 10 ** 7) -> dict[str, Any]:
-    """Jacobi from zero on the manufactured eigen-problem; error vs the FP64 solution alpha*v."""
+                  check_every: int = 100, max_sweeps: int = 10 ** 7,
+                  progress=None) -> dict[str, Any]:
+    """Jacobi from zero on the manufactured eigen-problem; error vs the FP64 solution alpha*v.
+
+    The relative residual of the exact iteration is ``mu^s`` with ``mu = 2 d c1 cos(pi h)``
+    (``c1`` the float32 coefficient), reported as ``exact_rel_residual`` beside the measured one
+    so the distance to the storage floor is visible.
+    """
     v, b, alpha = an.poisson_eigen_source(n, dims)
     u0 = np.zeros(v.shape, np.float32)
-    res = solve(key, u0, b.astype(np.float32), tol=tol, check_every=check_every,
-                max_sweeps=max_sweeps)
-    exact = alpha * v
+    b32 = b.astype(np.float32)
+    del b
+    res = solve(key, u0, b32, tol=tol, check_every=check_every, max_sweeps=max_sweeps,
+                progress=progress)
+    c1 = float(np.float32(0.5 / dims))
+    mu = 2 * dims * c1 * math.cos(math.pi / (n - 1))
+    err_fp64 = _rel(res.u.astype(np.float64), alpha * v)
+    err_pde = _rel(res.u.astype(np.float64), v)
     return {"variant": key, "storage": VARIANTS[key].storage, "n": n, "dims": dims,
             "status": res.status, "sweeps": res.sweeps, "rel_residual": res.rel_residual,
-            "seconds": res.seconds,
-            "predicted_sweeps": math.ceil(math.log(tol) / math.log(math.cos(math.pi / (n - 1)))),
-            "err_vs_fp64": _rel(res.u.astype(np.float64), exact),
-            "err_vs_pde": _rel(res.u.astype(np.float64), v)}
+            "exact_rel_residual": mu ** res.sweeps, "seconds": res.seconds,
+            "predicted_sweeps": math.ceil(math.log(tol) / math.log(mu)),
+            "err_vs_fp64": err_fp64, "err_vs_pde": err_pde,
+            "history": res.history[:: max(1, len(res.history) // 64)]}
 
 
 def run_all(quick: bool = True) -> dict[str, Any]:
