@@ -139,7 +139,7 @@ int ph3_graph_destroy(void* graph_exec);
  *
  * A solve is either a fixed number of steps (mode 0, heat) or Jacobi sweeps until the FP64
  * residual falls below tol relative to the initial one (mode 1), checked every
- * PH3_BATCH_CHECK sweeps. The state always lives in buf[done & 1]. Every decision (tier,
+ * PH3_BATCH_CHECK sweeps. The state lives in buf[(done / kpass) & 1]. Every decision (tier,
  * tile shape, check points, reduction order) depends only on the solve itself, so a solve
  * gives bit-identical results alone or in any batch.
  *
@@ -163,6 +163,10 @@ typedef struct ph3_solve_desc {
     const float* src;            /* optional float32 source b, or NULL */
     int mode;                    /* 0: run `steps` steps; 1: Jacobi to `tol` */
     int stall_window;            /* mode 1: stagnation window in checks (0 = off, max 31) */
+    int kpass;                   /* steps fused per tier-2 pass: 1, or for float32 storage 2 / 4
+                                    / 8 dividing PH3_BATCH_CHECK and `steps` (0 means 1). The
+                                    state lives in buf[(done / kpass) & 1]. */
+    int reserved;                /* must be 0 */
     long long steps;             /* mode 0: steps; mode 1: sweep cap */
     double tol;                  /* mode 1: target ||r|| / ||r0|| */
     double log_rate;             /* mode 1: ln(Jacobi spectral radius) * PH3_BATCH_CHECK (< 0) */

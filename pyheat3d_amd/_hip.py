@@ -93,7 +93,7 @@ class SolveDesc(ctypes.Structure):
         ("dims", c_int), ("nz", c_int), ("ny", c_int), ("nx", c_int),
         ("periodic", c_int), ("storage", c_int), ("c0", c_float), ("c1", c_float),
         ("buf", c_void_p * 2), ("src", c_void_p), ("mode", c_int), ("stall_window", c_int),
-        ("steps", ctypes.c_longlong), ("tol", c_double), ("log_rate", c_double),
+        ("kpass", c_int), ("reserved", c_int), ("steps", ctypes.c_longlong), ("tol", c_double), ("log_rate", c_double),
         ("stall_fraction", c_double), ("done", ctypes.c_longlong), ("res0_sq", c_double),
         ("res_sq", c_double), ("status", c_int), ("checks", c_int),
         ("ring", c_double * STALL_RING),

@@ -198,6 +198,11 @@ class Batch:
             self.handle = 0
 
 
+def current_buffer(d: _hip.SolveDesc) -> int:
+    """Index of the buffer holding a solve's state: passes of ``kpass`` steps alternate them."""
+    return (int(d.done) // max(1, int(d.kpass))) & 1
+
+
 def _result(req: Request, d: _hip.SolveDesc) -> Result:
     rel = None
     if req.mode == "jacobi":
@@ -252,7 +257,7 @@ class BatchSolver:
         for req, f, d in zip(self.requests, self.fields, self.batch.get()):
             res = _result(req, d)
             if keep_fields:
-                res.u = f.download(int(d.done) & 1, self.stream)
+                res.u = f.download(current_buffer(d), self.stream)
             out.append(res)
         return out
 

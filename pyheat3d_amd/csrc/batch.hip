@@ -13,8 +13,10 @@
 // All per-site arithmetic is the canonical update of ph3_common.h, so a grid gets the same bits
 // whichever tier, tile or batch it runs in; tier and check points depend only on the grid.
 #include "march_tiles.h"
+#include "tb_tiles.h"
 
 #include <math.h>
+#include <type_traits>
 #include <vector>
 
 using ph3host::set_error;
@@ -28,9 +30,18 @@ constexpr int kZch3 = 32;         // tier-2 3D z chunk
 constexpr int kYch2 = 64;         // tier-2 2D y chunk
 constexpr int kR3 = 4, kW3 = 4;   // tier-2 3D tile: 4 waves x 4 rows, 64 sites wide
 constexpr int kR2 = 4, kW2 = 4;   // tier-2 2D tile: 4 waves x 64 columns, 4 rows per step
+constexpr int kTbR3 = 8, kTbW3 = 4; // tier-2 3D temporal-blocking tile: 4 waves x 8 rows
 
 __device__ __forceinline__ long long sites_of(const ph3_solve_desc& d) {
     return (long long)d.nz * d.ny * d.nx;
+}
+
+// Steps fused per tier-2 pass of a solve (0 or 1 = single steps).
+__host__ __device__ inline int kpass_of(const ph3_solve_desc& d) { return d.kpass > 1 ? d.kpass : 1; }
+
+// Index of the buffer holding the current state.
+__host__ __device__ inline int cur_buf(const ph3_solve_desc& d) {
+    return (int)((d.done / kpass_of(d)) & 1);
 }
 
 // ---- one workgroup: sweep and residual of a whole grid ---------------------------------------
@@ -218,7 +229,7 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
         ph3_solve_desc* dp = descs + list[t];
         const ph3_solve_desc d = *dp;
         if (d.mode != 1 || d.checks != 0) continue;   // uniform over the workgroup
-        const double r = wg_residual<T>(d, d.buf[d.done & 1], sm);
+        const double r = wg_residual<T>(d, d.buf[cur_buf(d)], sm);
         if (threadIdx.x == 0) {
             dp->res0_sq = r;
             dp->res_sq = r;
@@ -231,16 +242,27 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 
 // ---- tier 2 ----------------------------------------------------------------------------------
 
-__host__ __device__ inline long long tiles_of(int dims, int nz, int ny, int nx) {
+// Tier-2 tiles of one grid: z-march / y-march tiles for single steps, overlapped
+// temporal-blocking tiles (3D: 64 x 32 extended, 2D: 4 strips of 64 columns) for K > 1.
+__host__ __device__ inline long long tiles_of(int dims, int nz, int ny, int nx, int K) {
+    if (K <= 1) {
+        if (dims == 3) {
+            const long long tx = (nx + 63) / 64, ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
+            return tx * ty * ((nz + kZch3 - 1) / kZch3);
+        }
+        const long long strips = (nx + 63) / 64;
+        return ((strips + kW2 - 1) / kW2) * ((ny + kYch2 - 1) / kYch2);
+    }
     if (dims == 3) {
-        const long long tx = (nx + 63) / 64, ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
+        const long long tx = (nx + 64 - 2 * K - 1) / (64 - 2 * K);
+        const long long ty = (ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
         return tx * ty * ((nz + kZch3 - 1) / kZch3);
     }
-    const long long strips = (nx + 63) / 64;
+    const long long strips = (nx + 64 - 2 * K - 1) / (64 - 2 * K);
     return ((strips + kW2 - 1) / kW2) * ((ny + kYch2 - 1) / kYch2);
 }
 
-template <typename T, bool P, bool S, int DIMS>
+template <typename T, bool P, bool S, int DIMS, int K>
 __global__ void __launch_bounds__(kWG)
 batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
            const long long* __restrict__ prefix, int n, int p) {
@@ -252,23 +274,39 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
             if (prefix[mid] <= t) lo = mid; else hi = mid - 1;
         }
         const ph3_solve_desc& d = descs[list[lo]];
-        if (d.status == 0 && d.done + p < d.steps) {
+        if (d.status == 0 && d.done + (long long)p * K < d.steps) {
             const long long local = t - prefix[lo];
-            const int cur = (int)((d.done + p) & 1);
+            const int cur = (int)((d.done / K + p) & 1);
             const T* u = (const T*)d.buf[cur];
             T* o = (T*)d.buf[cur ^ 1];
             if (DIMS == 3) {
-                const int tx = (d.nx + 63) / 64, ty = (d.ny + kR3 * kW3 - 1) / (kR3 * kW3);
-                const int ix = (int)(local % tx), rest = (int)(local / tx);
-                zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0, d.c1,
-                                                    kZch3, ix, rest % ty, rest / ty, 0);
+                if constexpr (K == 1) {
+                    const int tx = (d.nx + 63) / 64, ty = (d.ny + kR3 * kW3 - 1) / (kR3 * kW3);
+                    const int ix = (int)(local % tx), rest = (int)(local / tx);
+                    zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
+                                                        d.c1, kZch3, ix, rest % ty, rest / ty, 0);
+                } else {
+                    const int tx = (d.nx + 64 - 2 * K - 1) / (64 - 2 * K);
+                    const int ty = (d.ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
+                    const int ix = (int)(local % tx), rest = (int)(local / tx);
+                    tb3d_tile<T, P, S, 1, kTbR3, kTbW3, K>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
+                                                           d.c1, kZch3, ix, rest % ty, rest / ty);
+                }
             } else {
-                const int gxn = ((d.nx + 63) / 64 + kW2 - 1) / kW2;
-                ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1, kYch2,
-                                                    (int)(local % gxn), (int)(local / gxn));
+                if constexpr (K == 1) {
+                    const int gxn = ((d.nx + 63) / 64 + kW2 - 1) / kW2;
+                    ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1,
+                                                        kYch2, (int)(local % gxn),
+                                                        (int)(local / gxn));
+                } else {
+                    const int strips = (d.nx + 64 - 2 * K - 1) / (64 - 2 * K);
+                    const int gxn = (strips + kW2 - 1) / kW2;
+                    tb2d_tile<T, P, S, 1, K, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1, kYch2,
+                                                  (int)(local % gxn), (int)(local / gxn));
+                }
             }
         }
-        __syncthreads();  // the 3D tile's LDS slot is reused by the next tile
+        __syncthreads();  // the 3D tiles' LDS slot is reused by the next tile
     }
 }
 
@@ -293,7 +331,7 @@ batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
         const int t = item / kSlices, sl = item - t * kSlices;
         const ph3_solve_desc& d = descs[list[t]];
         if (d.mode != 1 || d.status != 0 || d.storage != Codec<T>::code) continue;
-        const double part = residual_rows<T>((const T*)d.buf[d.done & 1], d.src, d.dims, d.nz,
+        const double part = residual_rows<T>((const T*)d.buf[cur_buf(d)], d.src, d.dims, d.nz,
                                              d.ny, d.nx, d.periodic != 0, d.c0, d.c1,
                                              sl * (kWG / 64) + (threadIdx.x >> 6),
                                              kSlices * (kWG / 64), threadIdx.x & 63, 64);
@@ -318,7 +356,7 @@ __global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list,
 // ---- host side -----------------------------------------------------------------------------
 
 struct Tier2Class {
-    int storage, dims, periodic, src;
+    int storage, dims, periodic, src, kpass;
     std::vector<int> idx;
     int* d_list = nullptr;
     long long* d_prefix = nullptr;
@@ -395,11 +433,29 @@ static int check_desc(const ph3_solve_desc& d, int i) {
     if (d.storage < 0 || d.storage > 2) { set_error("solve %d: bad storage", i); return PH3_EINVAL; }
     if (!d.buf[0] || !d.buf[1] || d.buf[0] == d.buf[1]) { set_error("solve %d: buffers", i); return PH3_EINVAL; }
     if (d.mode != 0 && d.mode != 1) { set_error("solve %d: mode %d", i, d.mode); return PH3_EINVAL; }
+    if (d.kpass < 0 || d.kpass > 8 || (d.kpass & (d.kpass - 1))) {
+        set_error("solve %d: kpass %d not in {0, 1, 2, 4, 8}", i, d.kpass);
+        return PH3_EINVAL;
+    }
     if (d.steps < 0 || d.stall_window < 0 || d.stall_window >= PH3_STALL_RING) {
         set_error("solve %d: steps / stall_window out of range", i);
         return PH3_EINVAL;
     }
     return PH3_OK;
+}
+
+// Steps fused per tier-2 pass, a function of the solve alone: 1 for tier-1 grids and for 16-bit
+// storage (whose rounding cadence is part of the result); for float32 the largest of 8 (2D) or
+// 2 (3D) halvings that divides PH3_BATCH_CHECK and the step count. A caller's explicit kpass > 1
+// is kept when it is valid.
+static int choose_kpass(const ph3_solve_desc& d) {
+    const long long sites = (long long)d.nz * d.ny * d.nx;
+    if (sites <= PH3_BATCH_SMALL_SITES || d.storage != ph3::kF32) return 1;
+    int k = d.kpass > 1 ? d.kpass : (d.dims == 3 ? 2 : 8);
+    if (d.dims == 3 && k > 2) k = 2;
+    if (k > 8) k = 8;
+    while (k > 1 && (PH3_BATCH_CHECK % k || d.steps % k)) k >>= 1;
+    return k;
 }
 
 extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, void* stream) {
@@ -410,10 +466,12 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     PH3_CHECK(hipStreamSynchronize(s));
     free_set(b);
     b->n = n;
-    if (n) PH3_CHECK(hipMemcpyAsync(b->d_desc, descs, sizeof(ph3_solve_desc) * n, hipMemcpyHostToDevice, s));
+    std::vector<ph3_solve_desc> table(descs, descs + n);
+    for (auto& d : table) d.kpass = choose_kpass(d);
+    if (n) PH3_CHECK(hipMemcpyAsync(b->d_desc, table.data(), sizeof(ph3_solve_desc) * n, hipMemcpyHostToDevice, s));
     std::vector<int> small[3], init[3], large;
     for (int i = 0; i < n; ++i) {
-        const ph3_solve_desc& d = descs[i];
+        const ph3_solve_desc& d = table[i];
         init[d.storage].push_back(i);
         const long long sites = (long long)d.nz * d.ny * d.nx;
         if (sites <= PH3_BATCH_SMALL_SITES) { small[d.storage].push_back(i); continue; }
@@ -421,9 +479,10 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         Tier2Class* c = nullptr;
         for (auto& k : b->classes)
             if (k.storage == d.storage && k.dims == d.dims && k.periodic == (d.periodic != 0) &&
-                k.src == (d.src != nullptr)) c = &k;
+                k.src == (d.src != nullptr) && k.kpass == d.kpass) c = &k;
         if (!c) {
-            b->classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr, {}});
+            b->classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr,
+                                            d.kpass, {}});
             c = &b->classes.back();
         }
         c->idx.push_back(i);
@@ -439,8 +498,8 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     for (auto& c : b->classes) {
         std::vector<long long> prefix(c.idx.size() + 1, 0);
         for (size_t k = 0; k < c.idx.size(); ++k) {
-            const ph3_solve_desc& d = descs[c.idx[k]];
-            prefix[k + 1] = prefix[k] + ph3::tiles_of(d.dims, d.nz, d.ny, d.nx);
+            const ph3_solve_desc& d = table[c.idx[k]];
+            prefix[k + 1] = prefix[k] + ph3::tiles_of(d.dims, d.nz, d.ny, d.nx, d.kpass);
         }
         if (int r = upload(b, c.idx, &c.d_list, s)) return r;
         if (int r = upload(b, prefix, &c.d_prefix, s)) return r;
@@ -460,22 +519,33 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     return PH3_OK;
 }
 
-template <typename T, bool P, bool S, int DIMS>
+template <typename T, bool P, bool S, int DIMS, int K>
 static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
-    const unsigned g = (unsigned)ph3::kPassGroups;
-    ph3::batch_pass<T, P, S, DIMS><<<g, ph3::kWG, 0, s>>>(b->d_desc, c.d_list, c.d_prefix,
-                                                          (int)c.idx.size(), p);
+    ph3::batch_pass<T, P, S, DIMS, K><<<(unsigned)ph3::kPassGroups, ph3::kWG, 0, s>>>(
+        b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p);
+}
+
+template <typename T, bool P, bool S, int DIMS>
+static void pass_k(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (c.kpass == 2) return launch_pass<T, P, S, DIMS, 2>(c, b, p, s);
+        if constexpr (DIMS == 2) {
+            if (c.kpass == 4) return launch_pass<T, P, S, DIMS, 4>(c, b, p, s);
+            if (c.kpass == 8) return launch_pass<T, P, S, DIMS, 8>(c, b, p, s);
+        }
+    }
+    launch_pass<T, P, S, DIMS, 1>(c, b, p, s);
 }
 
 template <typename T>
 static void pass_class(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
     const bool P = c.periodic, S = c.src;
     if (c.dims == 3) {
-        if (P) { if (S) launch_pass<T, true, true, 3>(c, b, p, s); else launch_pass<T, true, false, 3>(c, b, p, s); }
-        else { if (S) launch_pass<T, false, true, 3>(c, b, p, s); else launch_pass<T, false, false, 3>(c, b, p, s); }
+        if (P) { if (S) pass_k<T, true, true, 3>(c, b, p, s); else pass_k<T, true, false, 3>(c, b, p, s); }
+        else { if (S) pass_k<T, false, true, 3>(c, b, p, s); else pass_k<T, false, false, 3>(c, b, p, s); }
     } else {
-        if (P) { if (S) launch_pass<T, true, true, 2>(c, b, p, s); else launch_pass<T, true, false, 2>(c, b, p, s); }
-        else { if (S) launch_pass<T, false, true, 2>(c, b, p, s); else launch_pass<T, false, false, 2>(c, b, p, s); }
+        if (P) { if (S) pass_k<T, true, true, 2>(c, b, p, s); else pass_k<T, true, false, 2>(c, b, p, s); }
+        else { if (S) pass_k<T, false, true, 2>(c, b, p, s); else pass_k<T, false, false, 2>(c, b, p, s); }
     }
 }
 
@@ -504,6 +574,7 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
         for (int seg = 0; seg < segs; ++seg) {
             for (int p = 0; p < PH3_BATCH_CHECK; ++p) {
                 for (const auto& c : b->classes) {
+                    if (p >= PH3_BATCH_CHECK / c.kpass) continue;  // k-step passes: fewer per segment
                     switch (c.storage) {
                         case ph3::kF32: pass_class<float>(c, b, p, s); break;
                         case ph3::kF16: pass_class<_Float16>(c, b, p, s); break;

@@ -19,187 +19,19 @@
 //
 // Dirichlet: sites on or outside the grid boundary are fixed (a stage keeps their value);
 // intermediate levels stay in float32 registers and only level k is rounded to storage.
-#include "sweep_impl.h"
-#include "vec_io.h"
+#include "tb_tiles.h"
 
 namespace ph3 {
-
-__device__ __forceinline__ int tb_logical_tile(int remap) {
-    const int L = blockIdx.x;
-    if (!remap) return L;
-    const int per = gridDim.x >> 3;
-    return (L & 7) * per + (L >> 3);
-}
-
-__device__ __forceinline__ int wrap_or_clamp(int i, int n, bool periodic) {
-    if (periodic) { i %= n; return i < 0 ? i + n : i; }
-    return i < 0 ? 0 : (i >= n ? n - 1 : i);
-}
-
-// First site of a V-wide lane vector at x = i (i % V == 0, n % V == 0): wrapped, or clamped
-// so that all V sites stay inside the row.
-template <int V>
-__device__ __forceinline__ int vec_col(int i, int n, bool periodic) {
-    if (periodic) { i %= n; return i < 0 ? i + n : i; }
-    return i < 0 ? 0 : (i > n - V ? n - V : i);
-}
-
-// halo of the extended tile in x, rounded up to whole lane vectors (keeps vector alignment)
-template <int V, int K> struct TbGeom {
-    static constexpr int KV = ((K + V - 1) / V) * V;
-    static constexpr int OX = 64 * V - 2 * KV;  // output sites per tile along x
-};
 
 template <typename T, bool P, bool S, int V, int R, int W, int K>
 __global__ void __launch_bounds__(64 * W)
 tb3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
      int nz, int ny, int nx, float c0, float c1, int zch, int tiles_x, int tiles_y, int remap)
 {
-    using G = TbGeom<V, K>;
-    constexpr int OY = W * R - 2 * K;  // output rows per tile
-    __shared__ float edge[2][W][2][64 * V];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
     const int L = tb_logical_tile(remap);
-    const int tx = L % tiles_x;
     const int rest = L / tiles_x;
-    const int ty = rest % tiles_y;
-    const int cz = rest / tiles_y;
-    const int z0 = cz * zch, z1 = min(z0 + zch, nz);
-    const int gx = tx * G::OX - G::KV + lane * V;   // first x site of this lane (may be outside)
-    const int gy0 = ty * OY - K + wave * R;         // first row of this wave (may be outside)
-    const int lx = vec_col<V>(gx, nx, P);           // vector-aligned: nx % V == 0, gx % V == 0
-    const size_t plane = (size_t)nx * ny;
-
-    size_t roff[R];
-    uint32_t fixmask = 0;      // bit r*V+e: site (row r, element e) is a fixed Dirichlet site
-    uint32_t wmask = 0;        // bit r*V+e: site is written (inside output region and grid)
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int gy = gy0 + r;
-        roff[r] = (size_t)wrap_or_clamp(gy, ny, P) * nx;
-        const int row = wave * R + r;
-        const bool row_out = row >= K && row < W * R - K && gy >= 0 && gy < ny;
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-            const int x = gx + e;
-            const int col = lane * V + e;
-            const bool fixed = !P && (x <= 0 || x >= nx - 1 || gy <= 0 || gy >= ny - 1);
-            if (fixed) fixmask |= 1u << (r * V + e);
-            if (row_out && col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx && !fixed)
-                wmask |= 1u << (r * V + e);
-        }
-    }
-
-    float q[K][3][R][V];   // level s: planes c-1, c, c+1 (c = plane computed by stage s+1)
-    float pref[R][V];
-#pragma unroll
-    for (int s = 0; s < K; ++s)
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-#pragma unroll
-                for (int e = 0; e < V; ++e) q[s][p][r][e] = 0.f;
-    {
-        const size_t zo = (size_t)wrap_or_clamp(z0 - K, nz, P) * plane;
-#pragma unroll
-        for (int r = 0; r < R; ++r) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
-    }
-    int par = 0;
-    for (int iz = z0 - K; iz < z1 + K; ++iz) {
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                q[0][0][r][e] = q[0][1][r][e];
-                q[0][1][r][e] = q[0][2][r][e];
-                q[0][2][r][e] = pref[r][e];
-            }
-        if (iz + 1 < z1 + K) {
-            const size_t zo = (size_t)wrap_or_clamp(iz + 1, nz, P) * plane;
-#pragma unroll
-            for (int r = 0; r < R; ++r) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
-        }
-        float res[R][V];
-#pragma unroll
-        for (int s = 1; s <= K; ++s) {
-            const int c = iz - s;                       // plane computed by this stage
-            const bool zfixed = !P && (c <= 0 || c >= nz - 1);
-            const float (&ctr)[R][V] = q[s - 1][1];
-            // rows of the neighbouring waves (level s-1, plane c)
-            float up[V], dn[V];
-            if (W > 1) {
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    edge[par][wave][0][lane * V + e] = ctr[0][e];
-                    edge[par][wave][1][lane * V + e] = ctr[R - 1][e];
-                }
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    up[e] = wave > 0 ? edge[par][wave - 1][1][lane * V + e] : ctr[0][e];
-                    dn[e] = wave < W - 1 ? edge[par][wave + 1][0][lane * V + e] : ctr[R - 1][e];
-                }
-                par ^= 1;
-            } else {
-#pragma unroll
-                for (int e = 0; e < V; ++e) { up[e] = ctr[0][e]; dn[e] = ctr[R - 1][e]; }
-            }
-            float bsrc[R][V];
-            if (S) {
-                const size_t zo = (size_t)wrap_or_clamp(c, nz, P) * plane;
-#pragma unroll
-                for (int r = 0; r < R; ++r) load_vec<float, V>(b, zo + roff[r] + lx, bsrc[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float fromL = __shfl_up(ctr[r][V - 1], 1);
-                const float fromR = __shfl_down(ctr[r][0], 1);
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    const float xm = e > 0 ? ctr[r][e - 1] : fromL;
-                    const float xp = e < V - 1 ? ctr[r][e + 1] : fromR;
-                    const float ym = r > 0 ? ctr[r - 1][e] : up[e];
-                    const float yp = r < R - 1 ? ctr[r + 1][e] : dn[e];
-                    float sum = xm + xp;
-                    sum = sum + (ym + yp);
-                    sum = sum + (q[s - 1][0][r][e] + q[s - 1][2][r][e]);
-                    float v = update(c0, c1, ctr[r][e], sum);
-                    if (S) v = v + bsrc[r][e];
-                    if (!P && (zfixed || ((fixmask >> (r * V + e)) & 1u))) v = ctr[r][e];
-                    res[r][e] = v;
-                }
-            }
-            if (s < K) {
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-#pragma unroll
-                    for (int e = 0; e < V; ++e) {
-                        q[s][0][r][e] = q[s][1][r][e];
-                        q[s][1][r][e] = q[s][2][r][e];
-                        q[s][2][r][e] = res[r][e];
-                    }
-            }
-        }
-        const int c = iz - K;
-        const bool zok = c >= z0 && c < z1 && (P || (c > 0 && c < nz - 1));
-        if (zok && wmask) {
-            const size_t zo = (size_t)c * plane;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                constexpr uint32_t full = (1u << V) - 1u;
-                const uint32_t m = (wmask >> (r * V)) & full;
-                if (m == full) {
-                    store_vec<T, V>(o, zo + roff[r] + gx, res[r]);
-                } else if (m) {
-#pragma unroll
-                    for (int e = 0; e < V; ++e)
-                        if ((m >> e) & 1u) st(o, zo + roff[r] + gx + e, res[r][e]);
-                }
-            }
-        }
-    }
+    tb3d_tile<T, P, S, V, R, W, K>(u, o, b, nz, ny, nx, c0, c1, zch, L % tiles_x,
+                                   rest % tiles_y, rest / tiles_y);
 }
 
 template <typename T, bool P, bool S, int V, int K, int W>
@@ -207,86 +39,8 @@ __global__ void __launch_bounds__(64 * W)
 tb2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
      int ny, int nx, float c0, float c1, int ych, int tiles_x, int remap)
 {
-    using G = TbGeom<V, K>;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
     const int L = tb_logical_tile(remap);
-    const int strip = (L % tiles_x) * W + wave;
-    const int cy = L / tiles_x;
-    const int y0 = cy * ych, y1 = min(y0 + ych, ny);
-    const int gx = strip * G::OX - G::KV + lane * V;
-    if (strip * G::OX - G::KV >= nx) return;          // whole wave past the grid (no barriers)
-    const int lx = vec_col<V>(gx, nx, P);
-    uint32_t fixmask = 0, wmask = 0;
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-        const int x = gx + e;
-        const int col = lane * V + e;
-        const bool fixed = !P && (x <= 0 || x >= nx - 1);
-        if (fixed) fixmask |= 1u << e;
-        if (col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx && !fixed) wmask |= 1u << e;
-    }
-    float q[K][3][V];
-    float pref[V];
-#pragma unroll
-    for (int s = 0; s < K; ++s)
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-            for (int e = 0; e < V; ++e) q[s][p][e] = 0.f;
-    load_vec<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx, pref);
-    for (int iy = y0 - K; iy < y1 + K; ++iy) {
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-            q[0][0][e] = q[0][1][e];
-            q[0][1][e] = q[0][2][e];
-            q[0][2][e] = pref[e];
-        }
-        if (iy + 1 < y1 + K)
-            load_vec<T, V>(u, (size_t)wrap_or_clamp(iy + 1, ny, P) * nx + lx, pref);
-        float res[V];
-#pragma unroll
-        for (int s = 1; s <= K; ++s) {
-            const int c = iy - s;
-            const bool yfixed = !P && (c <= 0 || c >= ny - 1);
-            const float (&ctr)[V] = q[s - 1][1];
-            float bsrc[V];
-            if (S) load_vec<float, V>(b, (size_t)wrap_or_clamp(c, ny, P) * nx + lx, bsrc);
-            const float fromL = __shfl_up(ctr[V - 1], 1);
-            const float fromR = __shfl_down(ctr[0], 1);
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                const float xm = e > 0 ? ctr[e - 1] : fromL;
-                const float xp = e < V - 1 ? ctr[e + 1] : fromR;
-                float sum = xm + xp;
-                sum = sum + (q[s - 1][0][e] + q[s - 1][2][e]);
-                float v = update(c0, c1, ctr[e], sum);
-                if (S) v = v + bsrc[e];
-                if (!P && (yfixed || ((fixmask >> e) & 1u))) v = ctr[e];
-                res[e] = v;
-            }
-            if (s < K) {
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    q[s][0][e] = q[s][1][e];
-                    q[s][1][e] = q[s][2][e];
-                    q[s][2][e] = res[e];
-                }
-            }
-        }
-        const int c = iy - K;
-        if (wmask && c >= y0 && c < y1 && (P || (c > 0 && c < ny - 1))) {
-            const size_t at = (size_t)c * nx + gx;
-            constexpr uint32_t full = (1u << V) - 1u;
-            if (wmask == full) {
-                store_vec<T, V>(o, at, res);
-            } else {
-#pragma unroll
-                for (int e = 0; e < V; ++e)
-                    if ((wmask >> e) & 1u) st(o, at + e, res[e]);
-            }
-        }
-    }
+    tb2d_tile<T, P, S, V, K, W>(u, o, b, ny, nx, c0, c1, ych, L % tiles_x, L / tiles_x);
 }
 
 // ---- launch ---------------------------------------------------------------------------------

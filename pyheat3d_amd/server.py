@@ -26,7 +26,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _hip
-from .batch import Batch, DeviceFields, Request, Result, make_desc, STATUS
+from .batch import STATUS, Batch, DeviceFields, Request, Result, current_buffer, make_desc
 
 
 def poisson_arrivals(n: int, rate: float, seed: int = 7) -> np.ndarray:
@@ -134,7 +134,7 @@ class Server:
                     res = Result(req.rid, STATUS[d.status], int(d.done), rel,
                                  seconds=t_done - arr)
                     if self.keep_fields:
-                        res.u = f.download(int(d.done) & 1, self.stream)
+                        res.u = f.download(current_buffer(d), self.stream)
                     f.close()
                     used -= req.device_bytes()
                     results[req.rid] = res
