@@ -213,10 +213,12 @@ def _result(req: Request, d: _hip.SolveDesc) -> Result:
 class BatchSolver:
     """All requests resident at once, advanced by graph-replayed batched rounds."""
 
-    def __init__(self, requests: list[Request], round_sweeps: int = 256, use_graph: bool = True):
+    def __init__(self, requests: list[Request], round_sweeps: int = 256, use_graph: bool = True,
+                 compact: bool = True):
         self.requests = requests
         self.round_sweeps = round_sweeps
         self.use_graph = use_graph
+        self.compact = compact
         self.stream = _hip.Stream()
         self.fields: list[DeviceFields] = []
         self.batch: Batch | None = None
@@ -225,36 +227,53 @@ class BatchSolver:
                 self.fields.append(DeviceFields(r, self.stream))
             self.stream.sync()
             self.batch = Batch(len(requests), self.stream)
-            self.batch.set([make_desc(r, f) for r, f in zip(requests, self.fields)])
+            self.state = [make_desc(r, f) for r, f in zip(requests, self.fields)]
+            self.table = list(range(len(requests)))  # request index of each table row
+            self.batch.set(self.state)
         except Exception:
             self.close()
             raise
 
     def run(self, max_rounds: int = 10 ** 6) -> dict:
-        """Replay rounds until every solve has finished; returns timing and round counts."""
+        """Replay rounds until every solve has finished; returns timing and round counts.
+
+        When half of the table has finished, the finished solves are dropped from the device
+        table (their state is kept on the host) and the round graph is re-recorded, so the
+        passes of the remaining solves do not walk the tiles of finished ones.
+        """
         graph = None
         t0 = time.perf_counter()
-        rounds = 0
+        rounds = compactions = 0
         try:
-            if self.use_graph:
-                graph = _hip.Graph(self.stream, lambda: self.batch.round(self.round_sweeps))
             while rounds < max_rounds:
+                if graph is None and self.use_graph:
+                    graph = _hip.Graph(self.stream, lambda: self.batch.round(self.round_sweeps))
                 if graph is not None:
                     graph.launch(self.stream)
                 else:
                     self.batch.round(self.round_sweeps)
                 rounds += 1
                 descs = self.batch.get()
-                if all(d.status != 0 for d in descs):
+                for row, d in zip(self.table, descs):
+                    self.state[row] = d
+                running = [row for row, d in zip(self.table, descs) if d.status == 0]
+                if not running:
                     break
+                if self.compact and len(running) <= len(self.table) // 2:
+                    self.table = running
+                    self.batch.set([self.state[row] for row in running])
+                    compactions += 1
+                    if graph is not None:
+                        graph.close()
+                        graph = None
         finally:
             if graph is not None:
                 graph.close()
-        return {"seconds": time.perf_counter() - t0, "rounds": rounds}
+        return {"seconds": time.perf_counter() - t0, "rounds": rounds, "compactions": compactions}
 
     def results(self, keep_fields: bool = False) -> list[Result]:
         out = []
-        for req, f, d in zip(self.requests, self.fields, self.batch.get()):
+        for req, f, d in zip(self.requests, self.fields, self.state):
             res = _result(req, d)
             if keep_fields:
                 res.u = f.download(current_buffer(d), self.stream)

@@ -26,8 +26,8 @@ namespace ph3 {
 constexpr int kWG = 256;          // lanes per workgroup in every batch kernel
 constexpr int kPassGroups = 2048; // workgroups of the persistent / looping launches
 constexpr int kSlices = 16;       // residual slices per tier-2 grid
-constexpr int kZch3 = 32;         // tier-2 3D z chunk
-constexpr int kYch2 = 64;         // tier-2 2D y chunk
+constexpr int kTier2Tiles = 512; // tier-2 tiles a large grid is spread over
+constexpr int kMaxChunk = 64;     // longest tier-2 march chunk
 constexpr int kR3 = 4, kW3 = 4;   // tier-2 3D tile: 4 waves x 4 rows, 64 sites wide
 constexpr int kR2 = 4, kW2 = 4;   // tier-2 2D tile: 4 waves x 64 columns, 4 rows per step
 constexpr int kTbR3 = 8, kTbW3 = 4; // tier-2 3D temporal-blocking tile: 4 waves x 8 rows
@@ -242,24 +242,41 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 
 // ---- tier 2 ----------------------------------------------------------------------------------
 
-// Tier-2 tiles of one grid: z-march / y-march tiles for single steps, overlapped
-// temporal-blocking tiles (3D: 64 x 32 extended, 2D: 4 strips of 64 columns) for K > 1.
-__host__ __device__ inline long long tiles_of(int dims, int nz, int ny, int nx, int K) {
-    if (K <= 1) {
-        if (dims == 3) {
-            const long long tx = (nx + 63) / 64, ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
-            return tx * ty * ((nz + kZch3 - 1) / kZch3);
-        }
-        const long long strips = (nx + 63) / 64;
-        return ((strips + kW2 - 1) / kW2) * ((ny + kYch2 - 1) / kYch2);
-    }
+// Tier-2 tiling of one grid: z-march / y-march tiles for single steps, overlapped
+// temporal-blocking tiles (3D: 64 x 32 extended, 2D: 4 strips of 64 columns) for K > 1. The
+// march chunk is chosen per grid so that a grid spreads over about kTier2Tiles tiles (a lone
+// large grid would otherwise run a few long, load-latency-bound marches), but no shorter than
+// max(8, 2K) planes / rows. Results do not depend on the tiling.
+struct T2Geom {
+    int tx, ty, chunk;  // tiles along x, along y (1 in 2D), march chunk
+    long long tiles;
+};
+
+__host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K) {
+    T2Geom g;
     if (dims == 3) {
-        const long long tx = (nx + 64 - 2 * K - 1) / (64 - 2 * K);
-        const long long ty = (ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
-        return tx * ty * ((nz + kZch3 - 1) / kZch3);
+        if (K <= 1) {
+            g.tx = (nx + 63) / 64;
+            g.ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
+        } else {
+            g.tx = (nx + 64 - 2 * K - 1) / (64 - 2 * K);
+            g.ty = (ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
+        }
+    } else {
+        const int ox = K <= 1 ? 64 : 64 - 2 * K;
+        g.tx = ((nx + ox - 1) / ox + kW2 - 1) / kW2;
+        g.ty = 1;
     }
-    const long long strips = (nx + 64 - 2 * K - 1) / (64 - 2 * K);
-    return ((strips + kW2 - 1) / kW2) * ((ny + kYch2 - 1) / kYch2);
+    const int march = dims == 3 ? nz : ny;
+    const long long txy = (long long)g.tx * g.ty;
+    const long long want = (kTier2Tiles + txy - 1) / txy;
+    int ch = (int)((march + want - 1) / want);
+    const int lo = K <= 1 ? 8 : (2 * K > 8 ? 2 * K : 8);
+    if (ch < lo) ch = lo;
+    if (ch > kMaxChunk) ch = kMaxChunk;
+    g.chunk = ch;
+    g.tiles = txy * ((march + ch - 1) / ch);
+    return g;
 }
 
 template <typename T, bool P, bool S, int DIMS, int K>
@@ -279,31 +296,24 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
             const int cur = (int)((d.done / K + p) & 1);
             const T* u = (const T*)d.buf[cur];
             T* o = (T*)d.buf[cur ^ 1];
+            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K);
+            const int ix = (int)(local % g.tx);
+            const long long rest = local / g.tx;
             if (DIMS == 3) {
-                if constexpr (K == 1) {
-                    const int tx = (d.nx + 63) / 64, ty = (d.ny + kR3 * kW3 - 1) / (kR3 * kW3);
-                    const int ix = (int)(local % tx), rest = (int)(local / tx);
+                const int iy = (int)(rest % g.ty), iz = (int)(rest / g.ty);
+                if constexpr (K == 1)
                     zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
-                                                        d.c1, kZch3, ix, rest % ty, rest / ty, 0);
-                } else {
-                    const int tx = (d.nx + 64 - 2 * K - 1) / (64 - 2 * K);
-                    const int ty = (d.ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
-                    const int ix = (int)(local % tx), rest = (int)(local / tx);
+                                                        d.c1, g.chunk, ix, iy, iz, 0);
+                else
                     tb3d_tile<T, P, S, 1, kTbR3, kTbW3, K>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
-                                                           d.c1, kZch3, ix, rest % ty, rest / ty);
-                }
+                                                           d.c1, g.chunk, ix, iy, iz);
             } else {
-                if constexpr (K == 1) {
-                    const int gxn = ((d.nx + 63) / 64 + kW2 - 1) / kW2;
+                if constexpr (K == 1)
                     ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1,
-                                                        kYch2, (int)(local % gxn),
-                                                        (int)(local / gxn));
-                } else {
-                    const int strips = (d.nx + 64 - 2 * K - 1) / (64 - 2 * K);
-                    const int gxn = (strips + kW2 - 1) / kW2;
-                    tb2d_tile<T, P, S, 1, K, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1, kYch2,
-                                                  (int)(local % gxn), (int)(local / gxn));
-                }
+                                                        g.chunk, ix, (int)rest);
+                else
+                    tb2d_tile<T, P, S, 1, K, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1, g.chunk,
+                                                  ix, (int)rest);
             }
         }
         __syncthreads();  // the 3D tiles' LDS slot is reused by the next tile
@@ -499,7 +509,7 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         std::vector<long long> prefix(c.idx.size() + 1, 0);
         for (size_t k = 0; k < c.idx.size(); ++k) {
             const ph3_solve_desc& d = table[c.idx[k]];
-            prefix[k + 1] = prefix[k] + ph3::tiles_of(d.dims, d.nz, d.ny, d.nx, d.kpass);
+            prefix[k + 1] = prefix[k] + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass).tiles;
         }
         if (int r = upload(b, c.idx, &c.d_list, s)) return r;
         if (int r = upload(b, prefix, &c.d_prefix, s)) return r;

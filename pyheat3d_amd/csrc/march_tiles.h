@@ -9,8 +9,8 @@ namespace ph3 {
 template <int V>
 __device__ __forceinline__ void x_neighbours(const float (&c)[V], float hl, float hr, bool useL,
                                              bool useR, float (&xm)[V], float (&xp)[V]) {
-    const float fromL = __shfl_up(c[V - 1], 1);
-    const float fromR = __shfl_down(c[0], 1);
+    const float fromL = from_prev_lane(c[V - 1]);
+    const float fromR = from_next_lane(c[0]);
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         xm[e] = e > 0 ? c[e - 1] : (useL ? hl : fromL);

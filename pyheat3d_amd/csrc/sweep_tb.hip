@@ -10,7 +10,7 @@
 // workgroup marches z through a (z, t) wavefront: at iteration iz, plane iz enters level 0 and
 // stage s (1..k) computes level s of plane iz - s from the 3-plane register queue of level
 // s-1. Rows across waves go through a double-buffered LDS slot (one barrier per stage); x
-// neighbours come from the lane itself or from lanes l-1 / l+1 by cross-lane shuffle. Level 0
+// neighbours come from the lane itself or from lanes l-1 / l+1 by a DPP wave shift. Level 0
 // is read from HBM once and level k written once per pass: 2 * sizeof(T) / k bytes per update
 // plus the halo over-fetch of the extended tile.
 //

@@ -7,7 +7,7 @@
 // 64*V*sizeof(T)-byte access per wave (16 B per lane when V*sizeof(T) = 16). Three planes
 // (z-1, z, z+1) sit in registers and plane z+2 is prefetched one iteration ahead.
 // Neighbours:
-//   x: inside the lane, else from lane l-1 / l+1 by a cross-lane shuffle, else (wave edges,
+//   x: inside the lane, else from lane l-1 / l+1 by a DPP wave shift, else (wave edges,
 //      grid edges) from a halo value loaded one plane ahead;
 //   y: rows of the same lane; across waves, each wave publishes its first and last row of
 //      plane z in a double-buffered LDS slot (one barrier per plane); only the rows just above

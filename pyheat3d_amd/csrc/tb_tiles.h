@@ -134,8 +134,8 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const float fromL = __shfl_up(ctr[r][V - 1], 1);
-                const float fromR = __shfl_down(ctr[r][0], 1);
+                const float fromL = from_prev_lane(ctr[r][V - 1]);
+                const float fromR = from_next_lane(ctr[r][0]);
 #pragma unroll
                 for (int e = 0; e < V; ++e) {
                     const float xm = e > 0 ? ctr[r][e - 1] : fromL;
@@ -231,8 +231,8 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             const float (&ctr)[V] = q[s - 1][1];
             float bsrc[V];
             if (S) load_vec<float, V>(b, (size_t)wrap_or_clamp(c, ny, P) * nx + lx, bsrc);
-            const float fromL = __shfl_up(ctr[V - 1], 1);
-            const float fromR = __shfl_down(ctr[0], 1);
+            const float fromL = from_prev_lane(ctr[V - 1]);
+            const float fromR = from_next_lane(ctr[0]);
 #pragma unroll
             for (int e = 0; e < V; ++e) {
                 const float xm = e > 0 ? ctr[e - 1] : fromL;
