@@ -88,112 +88,130 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
         return (size_t)zz * plane;
     };
 
-    float cm[R][V], cc[R][V], cn[R][V];
-    float yu[V], yd[V];      // memory halo rows of plane z (first / last wave only)
-    float xhl[R], xhr[R];    // x halos of plane z (used by edge lanes only)
-    float bs[R][V];          // source of plane z
+    // Plane queue in four register slots: at step k of an unrolled group of four planes, slot
+    // k holds plane z-1, k+1 plane z, k+2 plane z+1 and k+3 is free (slots mod 4). Rotating
+    // slot roles instead of values removes the per-plane register moves; halos and source
+    // double-buffer the same way with the parity of k.
+    // Loads run one plane ahead. Float32 rows land in the free slot directly. Everything that
+    // needs a conversion (16-bit rows, halo values) stays in its stored format until the step
+    // that uses it, so that no conversion right after a load makes the wave wait there.
+    constexpr bool kDirect = sizeof(T) == 4;
+    float q[4][R][V];
+    Packed<T, V> ahead[R];       // 16-bit storage: rows of the next plane, still packed
+    T xhl[2][R], xhr[2][R];      // x halos of plane z (used by edge lanes only)
+    Packed<T, V> ryu[2], ryd[2]; // memory halo rows of plane z (first / last wave only)
+    float bs[2][R][V];           // source of plane z
 
     {
         const size_t pm = pidx(z0 - 1), p0 = pidx(z0), p1 = pidx(z0 + 1);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            load_vec<T, V>(u, pm + roff[r] + xl, cm[r]);
-            load_vec<T, V>(u, p0 + roff[r] + xl, cc[r]);
-            load_vec<T, V>(u, p1 + roff[r] + xl, cn[r]);
-            xhl[r] = useL ? ld(u, p0 + roff[r] + hxl) : 0.f;
-            xhr[r] = useR ? ld(u, p0 + roff[r] + hxr) : 0.f;
-            if (S) load_vec<float, V>(b, p0 + roff[r] + xl, bs[r]);
+            load_vec<T, V>(u, pm + roff[r] + xl, q[0][r]);
+            load_vec<T, V>(u, p0 + roff[r] + xl, q[1][r]);
+            if constexpr (kDirect) load_vec<T, V>(u, p1 + roff[r] + xl, q[2][r]);
+            else ahead[r] = load_packed<T, V>(u, p1 + roff[r] + xl);
+            xhl[0][r] = useL ? u[p0 + roff[r] + hxl] : T{};
+            xhr[0][r] = useR ? u[p0 + roff[r] + hxr] : T{};
+            if (S) load_vec<float, V>(b, p0 + roff[r] + xl, bs[0][r]);
         }
-#pragma unroll
-        for (int e = 0; e < V; ++e) yu[e] = yd[e] = 0.f;
-        if (top) load_vec<T, V>(u, p0 + hup + xl, yu);
-        if (bot) load_vec<T, V>(u, p0 + hdn + xl, yd);
+        ryu[0] = ryu[1] = ryd[0] = ryd[1] = Packed<T, V>{};
+        if (top) ryu[0] = load_packed<T, V>(u, p0 + hup + xl);
+        if (bot) ryd[0] = load_packed<T, V>(u, p0 + hdn + xl);
     }
 
-    for (int z = z0; z < z1; ++z) {
-        // prefetch for plane z+1: its upper neighbour z+2, its halos and its source
-        float nn[R][V], nyu[V], nyd[V], nxl[R], nxr[R], nbs[R][V];
-        const bool more = z + 1 < z1;
-        if (more) {
-            const size_t p1 = pidx(z + 1), p2 = pidx(z + 2);
+    for (int zb = z0; zb < z1; zb += 4) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                load_vec<T, V>(u, p2 + roff[r] + xl, nn[r]);
-                nxl[r] = useL ? ld(u, p1 + roff[r] + hxl) : 0.f;
-                nxr[r] = useR ? ld(u, p1 + roff[r] + hxr) : 0.f;
-                if (S) load_vec<float, V>(b, p1 + roff[r] + xl, nbs[r]);
+        for (int k = 0; k < 4; ++k) {
+            const int z = zb + k;
+            if (z >= z1) break;                  // uniform over the workgroup
+            if constexpr (!kDirect) {
+                // plane z+1, loaded during the previous step
+#pragma unroll
+                for (int r = 0; r < R; ++r) unpack<T, V>(ahead[r], q[(k + 2) & 3][r]);
             }
-            if (top) load_vec<T, V>(u, p1 + hup + xl, nyu);
-            if (bot) load_vec<T, V>(u, p1 + hdn + xl, nyd);
-        }
+            const float (&cm)[R][V] = q[k & 3];
+            const float (&cc)[R][V] = q[(k + 1) & 3];
+            const float (&cn)[R][V] = q[(k + 2) & 3];
+            const int cur = k & 1, nxt = cur ^ 1;
 
-        // exchange the first / last rows of plane z between the waves of the workgroup
-        const int par = (z - z0) & 1;
-        if (W > 1) {
+            // prefetch for plane z+1: its upper neighbour z+2, its halos and its source
+            if (z + 1 < z1) {
+                const size_t p1 = pidx(z + 1), p2 = pidx(z + 2);
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
-                edge[par][wave][0][lane * V + e] = cc[0][e];
-                edge[par][wave][1][lane * V + e] = cc[R - 1][e];
-            }
-            __syncthreads();
-            if (!top) {
-#pragma unroll
-                for (int e = 0; e < V; ++e) yu[e] = edge[par][wave - 1][1][lane * V + e];
-            }
-            if (!bot) {
-#pragma unroll
-                for (int e = 0; e < V; ++e) yd[e] = edge[par][wave + 1][0][lane * V + e];
-            }
-        }
-
-        const size_t pz = (size_t)z * plane;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float xm[V], xp[V];
-            x_neighbours<V>(cc[r], xhl[r], xhr[r], useL, useR, xm, xp);
-            float w[V];
-#pragma unroll
-            for (int e = 0; e < V; ++e) {
-                const float ym = r > 0 ? cc[r - 1][e] : yu[e];
-                const float yp = r < R - 1 ? cc[r + 1][e] : yd[e];
-                float s = xm[e] + xp[e];
-                s = s + (ym + yp);
-                s = s + (cm[r][e] + cn[r][e]);
-                w[e] = update(c0, c1, cc[r][e], s);
-                if (S) w[e] = w[e] + bs[r][e];
-            }
-            if (xin && rstore[r]) {
-                const size_t at = pz + roff[r] + x;
-                if (P || (x > 0 && x + V < nx)) {
-                    store_out<T, V>(o, at, w, nt);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < V; ++e)
-                        if (x + e > 0 && x + e < nx - 1) st(o, at + e, w[e]);
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (kDirect) load_vec<T, V>(u, p2 + roff[r] + xl, q[(k + 3) & 3][r]);
+                    else ahead[r] = load_packed<T, V>(u, p2 + roff[r] + xl);
+                    xhl[nxt][r] = useL ? u[p1 + roff[r] + hxl] : T{};
+                    xhr[nxt][r] = useR ? u[p1 + roff[r] + hxr] : T{};
+                    if (S) load_vec<float, V>(b, p1 + roff[r] + xl, bs[nxt][r]);
                 }
+                if (top) ryu[nxt] = load_packed<T, V>(u, p1 + hup + xl);
+                if (bot) ryd[nxt] = load_packed<T, V>(u, p1 + hdn + xl);
             }
-        }
 
-        if (more) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                xhl[r] = nxl[r];
-                xhr[r] = nxr[r];
+            // rows above / below the wave: memory halo rows for the first / last wave, else
+            // the neighbouring waves' first / last rows of plane z exchanged through LDS
+            // (LDS parity (z - z0) & 1 == k & 1 since groups start at multiples of four)
+            float yu[V], yd[V];
+            unpack<T, V>(ryu[cur], yu);
+            unpack<T, V>(ryd[cur], yd);
+            if (W > 1) {
 #pragma unroll
                 for (int e = 0; e < V; ++e) {
-                    cm[r][e] = cc[r][e];
-                    cc[r][e] = cn[r][e];
-                    cn[r][e] = nn[r][e];
-                    if (S) bs[r][e] = nbs[r][e];
+                    edge[cur][wave][0][lane * V + e] = cc[0][e];
+                    edge[cur][wave][1][lane * V + e] = cc[R - 1][e];
+                }
+                __syncthreads();
+                if constexpr (kDirect) {
+                    if (!top) {
+#pragma unroll
+                        for (int e = 0; e < V; ++e) yu[e] = edge[cur][wave - 1][1][lane * V + e];
+                    }
+                    if (!bot) {
+#pragma unroll
+                        for (int e = 0; e < V; ++e) yd[e] = edge[cur][wave + 1][0][lane * V + e];
+                    }
+                } else {
+                    // 16-bit storage measured faster with a read by every wave (its own slot
+                    // at the tile edges) and a select instead of a branch
+                    const int wu = top ? 0 : wave - 1, wd = bot ? W - 1 : wave + 1;
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        const float eu = edge[cur][wu][1][lane * V + e];
+                        const float ed = edge[cur][wd][0][lane * V + e];
+                        yu[e] = top ? yu[e] : eu;
+                        yd[e] = bot ? yd[e] : ed;
+                    }
                 }
             }
-            if (top) {
+
+            const size_t pz = (size_t)z * plane;
 #pragma unroll
-                for (int e = 0; e < V; ++e) yu[e] = nyu[e];
-            }
-            if (bot) {
+            for (int r = 0; r < R; ++r) {
+                float xm[V], xp[V];
+                x_neighbours<V>(cc[r], Codec<T>::load(xhl[cur][r]), Codec<T>::load(xhr[cur][r]),
+                                useL, useR, xm, xp);
+                float w[V];
 #pragma unroll
-                for (int e = 0; e < V; ++e) yd[e] = nyd[e];
+                for (int e = 0; e < V; ++e) {
+                    const float ym = r > 0 ? cc[r - 1][e] : yu[e];
+                    const float yp = r < R - 1 ? cc[r + 1][e] : yd[e];
+                    float s = xm[e] + xp[e];
+                    s = s + (ym + yp);
+                    s = s + (cm[r][e] + cn[r][e]);
+                    w[e] = update(c0, c1, cc[r][e], s);
+                    if (S) w[e] = w[e] + bs[cur][r][e];
+                }
+                if (xin && rstore[r]) {
+                    const size_t at = pz + roff[r] + x;
+                    if (P || (x > 0 && x + V < nx)) {
+                        store_out<T, V>(o, at, w, nt);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < V; ++e)
+                            if (x + e > 0 && x + e < nx - 1) st(o, at + e, w[e]);
+                    }
+                }
             }
         }
     }
@@ -228,30 +246,37 @@ ymarch2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
         return (size_t)yy * nx;
     };
 
-    // win[q] holds row y0 - 1 + q, q = 0 .. R+1
+    // win[q] holds row y0 - 1 + q, q = 0 .. R+1. Prefetched 16-bit rows and all halo values
+    // stay in their stored format until the window advances (a conversion right after a load
+    // would make the wave wait for it there).
+    constexpr bool kDirect = sizeof(T) == 4;
     float win[R + 2][V];
-    float xhl[R], xhr[R], bs[R][V];
+    T xhl[R], xhr[R];
+    float bs[R][V];
 #pragma unroll
     for (int q = 0; q < R + 2; ++q) load_vec<T, V>(u, ridx(y0 - 1 + q) + xl, win[q]);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const size_t ro = ridx(y0 + r);
-        xhl[r] = useL ? ld(u, ro + hxl) : 0.f;
-        xhr[r] = useR ? ld(u, ro + hxr) : 0.f;
+        xhl[r] = useL ? u[ro + hxl] : T{};
+        xhr[r] = useR ? u[ro + hxr] : T{};
         if (S) load_vec<float, V>(b, ro + xl, bs[r]);
     }
 
     for (int y = y0; y < y1; y += R) {
         // prefetch the next window's R new rows (y+R+1 .. y+2R), halos and source of y+R ..
-        float nw[R][V], nxl[R], nxr[R], nbs[R][V];
+        float nw[R][V], nbs[R][V];
+        Packed<T, V> nwp[R];
+        T nxl[R], nxr[R];
         const bool more = y + R < y1;
         if (more) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                load_vec<T, V>(u, ridx(y + R + 1 + r) + xl, nw[r]);
+                if constexpr (kDirect) load_vec<T, V>(u, ridx(y + R + 1 + r) + xl, nw[r]);
+                else nwp[r] = load_packed<T, V>(u, ridx(y + R + 1 + r) + xl);
                 const size_t ro = ridx(y + R + r);
-                nxl[r] = useL ? ld(u, ro + hxl) : 0.f;
-                nxr[r] = useR ? ld(u, ro + hxr) : 0.f;
+                nxl[r] = useL ? u[ro + hxl] : T{};
+                nxr[r] = useR ? u[ro + hxr] : T{};
                 if (S) load_vec<float, V>(b, ro + xl, nbs[r]);
             }
         }
@@ -259,7 +284,8 @@ ymarch2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
         for (int r = 0; r < R; ++r) {
             const int yr = y + r;
             float xm[V], xp[V];
-            x_neighbours<V>(win[r + 1], xhl[r], xhr[r], useL, useR, xm, xp);
+            x_neighbours<V>(win[r + 1], Codec<T>::load(xhl[r]), Codec<T>::load(xhr[r]), useL,
+                            useR, xm, xp);
             float w[V];
 #pragma unroll
             for (int e = 0; e < V; ++e) {
@@ -289,11 +315,15 @@ ymarch2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
             for (int r = 0; r < R; ++r) {
                 xhl[r] = nxl[r];
                 xhr[r] = nxr[r];
+                if constexpr (kDirect) {
 #pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    win[r + 2][e] = nw[r][e];
-                    if (S) bs[r][e] = nbs[r][e];
+                    for (int e = 0; e < V; ++e) win[r + 2][e] = nw[r][e];
+                } else {
+                    unpack<T, V>(nwp[r], win[r + 2]);
                 }
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (S) bs[r][e] = nbs[r][e];
             }
         }
     }

@@ -66,14 +66,15 @@ __device__ __forceinline__ float update(float c0, float c1, float u, float s) {
 
 // Whole-wave lane shifts by one lane with DPP (wave_shr:1 / wave_shl:1, GFX9-family controls
 // available on gfx950): a VALU move instead of an LDS-crossbar permute. Lane 0 (resp. 63) has
-// no source lane and receives 0; callers use their own halo value there.
+// no source lane; with bound_ctrl it receives 0 and no register has to be zeroed first.
+// Callers use their own halo value there or never store what depends on it.
 __device__ __forceinline__ float from_prev_lane(float v) {  // lane i gets lane i-1
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-        0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(
+        __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_next_lane(float v) {  // lane i gets lane i+1
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-        0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(
+        __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ int wrap_lo(int i, int n) { return i < 0 ? i + n : i; }

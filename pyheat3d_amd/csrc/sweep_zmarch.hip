@@ -45,10 +45,10 @@ ymarch2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b
 }
 
 static int pick_vec(const SweepArgs& a, int elem_bytes) {
-    // widest lane vector (4 sites: 16 B in float32, 8 B in 16-bit storage, which measured
-    // faster than 16 B there) that divides nx and keeps every row aligned
+    // widest lane vector up to 16 bytes (4 float32 or 8 16-bit sites) that divides nx and
+    // keeps every row aligned
     const uintptr_t addr = (uintptr_t)a.u | (uintptr_t)a.o;
-    for (int v = 4; v > 1; v >>= 1) {
+    for (int v = 16 / elem_bytes; v > 1; v >>= 1) {
         const bool aligned = (addr % (size_t)(v * elem_bytes)) == 0 &&
                              (!a.b || ((uintptr_t)a.b % (size_t)(v * 4)) == 0);
         if (a.nx % v == 0 && aligned) return v;
@@ -101,10 +101,11 @@ static int ymarch_launch2(const SweepArgs& a) {
 template <typename T, bool P, bool S, int V>
 static int zmarch_shape3(const SweepArgs& a) {
     // Defaults measured on MI355X at 1024^3 (rows per wave, waves): float32 with 16-byte lanes
-    // (4, 8) 691 GLUPS; narrower lanes and 16-bit storage do best with (1, 16) or (4, 4).
+    // (4, 8) 705 GLUPS; 16-bit storage with 16-byte lanes (4, 4) 1300 GLUPS; narrower lanes
+    // (1, 16) or (4, 4).
     int dr = 4, dw = 4;
     if (sizeof(T) == 4 && V == 4) { dr = 4; dw = 8; }
-    else if ((sizeof(T) == 4 && V == 2) || (sizeof(T) == 2 && V >= 4)) { dr = 1; dw = 16; }
+    else if ((sizeof(T) == 4 && V == 2) || (sizeof(T) == 2 && V == 4)) { dr = 1; dw = 16; }
     const int R = a.tile.ty ? a.tile.ty : dr;
     const int W = a.tile.waves ? a.tile.waves : dw;
 #define PH3_ZM(r, w) if (R == r && W == w) return zmarch_launch3<T, P, S, V, r, w>(a)

@@ -71,8 +71,17 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         }
     }
 
-    float q[K][3][R][V];   // level s: planes c-1, c, c+1 (c = plane computed by stage s+1)
+    // Plane queues in three register slots per level: at march step i (i = iz - z0 + K) level
+    // s holds planes c-1, c, c+1 in slots i, i+1, i+2 (mod 3). A new value (the prefetched
+    // plane at level 0, the stage result at level s >= 1) overwrites slot i+2, whose plane c-2
+    // is no longer needed. Rotating slot roles over an unrolled group of three steps replaces
+    // the per-step register moves of a shifted queue.
+    float q[K][3][R][V];
+    // the next plane, loaded one step ahead: float32 as values, 16-bit storage still packed
+    // (a conversion right after the load would make the wave wait for it there)
+    constexpr bool kDirect = sizeof(T) == 4;
     float pref[R][V];
+    Packed<T, V> prefp[R];
 #pragma unroll
     for (int s = 0; s < K; ++s)
 #pragma unroll
@@ -84,29 +93,43 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     {
         const size_t zo = (size_t)wrap_or_clamp(z0 - K, nz, P) * plane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
+        for (int r = 0; r < R; ++r) {
+            if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
+            else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
+        }
     }
-    int par = 0;
-    for (int iz = z0 - K; iz < z1 + K; ++iz) {
+    int par = 0;   // LDS slot parity, flipped once per stage
+    for (int ib = z0 - K; ib < z1 + K; ib += 3) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+    for (int k = 0; k < 3; ++k) {
+        const int iz = ib + k;
+        if (iz >= z1 + K) break;                      // uniform over the workgroup
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
-                q[0][0][r][e] = q[0][1][r][e];
-                q[0][1][r][e] = q[0][2][r][e];
-                q[0][2][r][e] = pref[r][e];
+        for (int r = 0; r < R; ++r) {
+            if constexpr (kDirect) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) q[0][(k + 2) % 3][r][e] = pref[r][e];
+            } else {
+                unpack<T, V>(prefp[r], q[0][(k + 2) % 3][r]);
             }
+        }
         if (iz + 1 < z1 + K) {
             const size_t zo = (size_t)wrap_or_clamp(iz + 1, nz, P) * plane;
 #pragma unroll
-            for (int r = 0; r < R; ++r) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
+            for (int r = 0; r < R; ++r) {
+                if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
+                else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
+            }
         }
         float res[R][V];
 #pragma unroll
         for (int s = 1; s <= K; ++s) {
             const int c = iz - s;                       // plane computed by this stage
             const bool zfixed = !P && (c <= 0 || c >= nz - 1);
-            const float (&ctr)[R][V] = q[s - 1][1];
+            const float (&qm)[R][V] = q[s - 1][k];
+            const float (&ctr)[R][V] = q[s - 1][(k + 1) % 3];
+            const float (&qp)[R][V] = q[s - 1][(k + 2) % 3];
+            float (&dst)[R][V] = s < K ? q[s < K ? s : 0][(k + 2) % 3] : res;
             // rows of the neighbouring waves (level s-1, plane c)
             float up[V], dn[V];
             if (W > 1) {
@@ -144,22 +167,12 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                     const float yp = r < R - 1 ? ctr[r + 1][e] : dn[e];
                     float sum = xm + xp;
                     sum = sum + (ym + yp);
-                    sum = sum + (q[s - 1][0][r][e] + q[s - 1][2][r][e]);
+                    sum = sum + (qm[r][e] + qp[r][e]);
                     float v = update(c0, c1, ctr[r][e], sum);
                     if (S) v = v + bsrc[r][e];
                     if (!P && (zfixed || ((fixmask >> (r * V + e)) & 1u))) v = ctr[r][e];
-                    res[r][e] = v;
+                    dst[r][e] = v;
                 }
-            }
-            if (s < K) {
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-#pragma unroll
-                    for (int e = 0; e < V; ++e) {
-                        q[s][0][r][e] = q[s][1][r][e];
-                        q[s][1][r][e] = q[s][2][r][e];
-                        q[s][2][r][e] = res[r][e];
-                    }
             }
         }
         const int c = iz - K;
@@ -179,6 +192,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                 }
             }
         }
+    }
     }
 }
 
@@ -205,30 +219,45 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         if (fixed) fixmask |= 1u << e;
         if (col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx && !fixed) wmask |= 1u << e;
     }
+    // row queues rotated by slot as in tb3d_tile (slots i, i+1, i+2 mod 3 hold rows c-1, c,
+    // c+1 of level s at march step i)
     float q[K][3][V];
+    constexpr bool kDirect = sizeof(T) == 4;   // 16-bit rows stay packed until used
     float pref[V];
+    Packed<T, V> prefp;
 #pragma unroll
     for (int s = 0; s < K; ++s)
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
             for (int e = 0; e < V; ++e) q[s][p][e] = 0.f;
-    load_vec<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx, pref);
-    for (int iy = y0 - K; iy < y1 + K; ++iy) {
+    if constexpr (kDirect) load_vec<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx, pref);
+    else prefp = load_packed<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx);
+    for (int ib = y0 - K; ib < y1 + K; ib += 3) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) {
-            q[0][0][e] = q[0][1][e];
-            q[0][1][e] = q[0][2][e];
-            q[0][2][e] = pref[e];
+    for (int k = 0; k < 3; ++k) {
+        const int iy = ib + k;
+        if (iy >= y1 + K) break;
+        if constexpr (kDirect) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) q[0][(k + 2) % 3][e] = pref[e];
+        } else {
+            unpack<T, V>(prefp, q[0][(k + 2) % 3]);
         }
-        if (iy + 1 < y1 + K)
-            load_vec<T, V>(u, (size_t)wrap_or_clamp(iy + 1, ny, P) * nx + lx, pref);
+        if (iy + 1 < y1 + K) {
+            const size_t ro = (size_t)wrap_or_clamp(iy + 1, ny, P) * nx + lx;
+            if constexpr (kDirect) load_vec<T, V>(u, ro, pref);
+            else prefp = load_packed<T, V>(u, ro);
+        }
         float res[V];
 #pragma unroll
         for (int s = 1; s <= K; ++s) {
             const int c = iy - s;
             const bool yfixed = !P && (c <= 0 || c >= ny - 1);
-            const float (&ctr)[V] = q[s - 1][1];
+            const float (&qm)[V] = q[s - 1][k];
+            const float (&ctr)[V] = q[s - 1][(k + 1) % 3];
+            const float (&qp)[V] = q[s - 1][(k + 2) % 3];
+            float (&dst)[V] = s < K ? q[s < K ? s : 0][(k + 2) % 3] : res;
             float bsrc[V];
             if (S) load_vec<float, V>(b, (size_t)wrap_or_clamp(c, ny, P) * nx + lx, bsrc);
             const float fromL = from_prev_lane(ctr[V - 1]);
@@ -238,19 +267,11 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                 const float xm = e > 0 ? ctr[e - 1] : fromL;
                 const float xp = e < V - 1 ? ctr[e + 1] : fromR;
                 float sum = xm + xp;
-                sum = sum + (q[s - 1][0][e] + q[s - 1][2][e]);
+                sum = sum + (qm[e] + qp[e]);
                 float v = update(c0, c1, ctr[e], sum);
                 if (S) v = v + bsrc[e];
                 if (!P && (yfixed || ((fixmask >> e) & 1u))) v = ctr[e];
-                res[e] = v;
-            }
-            if (s < K) {
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    q[s][0][e] = q[s][1][e];
-                    q[s][1][e] = q[s][2][e];
-                    q[s][2][e] = res[e];
-                }
+                dst[e] = v;
             }
         }
         const int c = iy - K;
@@ -265,6 +286,7 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                     if ((wmask >> e) & 1u) st(o, at + e, res[e]);
             }
         }
+    }
     }
 }
 

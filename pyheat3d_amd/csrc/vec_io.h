@@ -42,6 +42,23 @@ __device__ __forceinline__ void store_vec(T* __restrict__ p, size_t i, const flo
     }
 }
 
+// A lane vector kept in its stored format between the load and the step that uses it: a
+// conversion placed right after a load would make the wave wait for that load there.
+template <typename T, int V> using Packed = typename RawVec<V * sizeof(T)>::type;
+
+template <typename T, int V>
+__device__ __forceinline__ Packed<T, V> load_packed(const T* __restrict__ p, size_t i) {
+    return *reinterpret_cast<const Packed<T, V>*>(p + i);
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void unpack(const Packed<T, V>& raw, float (&out)[V]) {
+    T tmp[V];
+    __builtin_memcpy(tmp, &raw, sizeof(raw));
+#pragma unroll
+    for (int e = 0; e < V; ++e) out[e] = Codec<T>::load(tmp[e]);
+}
+
 template <int V>
 __device__ __forceinline__ void load_vec_f32(const float* __restrict__ p, size_t i, float (&out)[V]) {
     load_vec<float, V>(p, i, out);

@@ -70,7 +70,7 @@ def _build() -> dict[str, Variant]:
         v.append(Variant("hip-cdna4", "zmarch" + _SHORT[st], _hip("zmarch", st), dims=(3,),
                          storage=st, source=True, rung=4,
                          description=f"HIP 2.5D z-march: register plane queue, 128-bit rows, "
-                                     f"shuffle x-halo, {st} storage"))
+                                     f"DPP x-halo, {st} storage"))
     for st in _ST:
         v.append(Variant("hip-cdna4", "ymarch" + _SHORT[st], _hip("tb", st, 1), dims=(2,),
                          storage=st, source=True, rung=4,

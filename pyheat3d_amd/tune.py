@@ -57,14 +57,16 @@ def lookup(kind: str, k: int, dims: int, storage: str, bc: str,
     return _hip.Tile(t["ty"], t["zch"], t["vec"], t["waves"], t.get("flags", 0))
 
 
-def candidates(kind: str, k: int, dims: int, nx: int) -> list[_hip.Tile]:
+def candidates(kind: str, k: int, dims: int, nx: int, storage: str = "float32") -> list[_hip.Tile]:
+    # lane vectors up to 16 bytes: 4 float32 or 8 16-bit sites
+    vecs = (1, 2, 4) if storage == "float32" else (1, 2, 4, 8)
     out = []
     if kind == "zmarch" and dims == 3:
-        for (r, w), v, z in itertools.product(ZMARCH_SHAPES, (1, 2, 4), (0, 128, 256)):
+        for (r, w), v, z in itertools.product(ZMARCH_SHAPES, vecs, (0, 128, 256)):
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, w, 0))
     elif kind == "zmarch":
-        for r, v, z in itertools.product((1, 2, 4, 8), (1, 2, 4), (0, 256)):
+        for r, v, z in itertools.product((1, 2, 4, 8), vecs, (0, 256)):
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, 4, 0))
     elif kind == "tb" and dims == 3:
@@ -72,7 +74,7 @@ def candidates(kind: str, k: int, dims: int, nx: int) -> list[_hip.Tile]:
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, w, 0))
     elif kind == "tb":
-        for v, z in itertools.product((1, 2, 4), (0, 128, 512)):
+        for v, z in itertools.product(vecs, (0, 128, 512)):
             if nx % v == 0:
                 out.append(_hip.Tile(0, z, v, 4, 0))
     return out
@@ -92,7 +94,7 @@ def tune(key: str, shape: tuple[int, ...], bc: str = "periodic", steps: int = 20
     ev0, ev1 = _hip.Event(), _hip.Event()
     best = None
     try:
-        for tile in candidates(kind, k, dims, shape[-1]):
+        for tile in candidates(kind, k, dims, shape[-1], var.storage):
             st.tile_override = tile
             try:
                 st.step(2 * k)
