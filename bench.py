@@ -12,6 +12,8 @@ Timing: W untimed steps, then exactly K steps between two synchronisations; ``va
 lattice updates per second over that region (GLUPS, inputs resident in HBM). The dominant
 kernel's average launch time comes from HIP events recorded on the stepper's own stream around
 the same region. ``cpu_baseline`` times the NumPy oracle (one core) on a bounded sample.
+``batch`` and ``serve`` carry the solves/s and p99 half of the metric on a bounded sample of the
+mixed batch workload (``--batch-n``; see ``batch_sample``).
 """
 
 from __future__ import annotations
@@ -137,6 +139,40 @@ def cpu_baseline(n: int, min_time: float) -> dict:
                       f"({os.cpu_count()} cores visible)"}
 
 
+def batch_sample(n: int) -> tuple[dict, dict]:
+    """The solves/s + p99 half of the metric on a bounded sample of the batch mix.
+
+    ``n`` mixed solves (60% 2D N in [64, 256], 40% 3D N in [32, 64], half 200 heat steps, half
+    Jacobi Laplace to a 1e-6 relative FP64 residual, float32): all submitted at once, batched
+    graph rounds vs one launch per sweep per solve (every 16th solve), then the same mix served
+    at 1000 Poisson arrivals per second for latency percentiles. The full-size mixes are in
+    ``python -m pyheat3d_amd batch|serve``.
+    """
+    from pyheat3d_amd.benchmark import batch_benchmark, serve_benchmark
+
+    n2, n3 = (64, 256), (32, 64)
+    t0 = time.perf_counter()
+    b = batch_benchmark(n, seed=2024, n2=n2, n3=n3, frac_jacobi=0.5, baseline_every=16)
+    log(f"batch: {b['batched']['solves_per_s']:.1f} solves/s batched vs "
+        f"{b['per_solve']['solves_per_s']:.2f} per-solve ({b['speedup']:.1f}x) in "
+        f"{time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    s = serve_benchmark(n, rate=1000.0, seed=2024, n2=n2, n3=n3)
+    log(f"serve: {s['solves_per_s']:.1f} solves/s, p50 {s['p50_ms']:.0f} ms, "
+        f"p99 {s['p99_ms']:.0f} ms in {time.perf_counter() - t0:.1f} s")
+    batch = {"workload": b["workload"],
+             "batched_solves_per_s": round(b["batched"]["solves_per_s"], 2),
+             "per_solve_solves_per_s": round(b["per_solve"]["solves_per_s"], 3),
+             "per_solve_sample": b["per_solve"]["sample"],
+             "speedup": round(b["speedup"], 2),
+             "jacobi": b["jacobi"],
+             "baseline_agrees_on_steps_and_status": b["baseline_agrees_on_steps_and_status"]}
+    serve = {"workload": s["workload"], "solves_per_s": round(s["solves_per_s"], 2),
+             "p50_ms": round(s["p50_ms"], 1), "p99_ms": round(s["p99_ms"], 1),
+             "statuses": s["statuses"]}
+    return batch, serve
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -151,6 +187,8 @@ def main() -> int:
     ap.add_argument("--cpu-min-time", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth probe")
+    ap.add_argument("--batch-n", type=int, default=512,
+                    help="solves in the batched-vs-per-solve and served sample (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -202,6 +240,9 @@ def main() -> int:
     pmc = load_traffic(f"zmarch_{a.n}")
     if pmc and pmc.get("variant") == a.variant and pmc.get("n") == a.n:
         traffic = round(pmc["hbm_bytes_per_launch"])
+    batch = serve = None
+    if a.batch_n > 0:
+        batch, serve = batch_sample(a.batch_n)
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_n, a.cpu_min_time)
 
     line = {
@@ -224,6 +265,8 @@ def main() -> int:
                   "what": "amplitude of each sine mode after the run vs the closed form g_m^steps"},
         "host_transfer": head["host_transfer"],
         "rungs": others,
+        "batch": batch,
+        "serve": serve,
         "device": {"name": info["name"], "arch": info["arch"], "cu_count": info["cu_count"]},
     }
     print(json.dumps(line), flush=True)
