@@ -13,7 +13,8 @@ lattice updates per second over that region (GLUPS, inputs resident in HBM). The
 kernel's average launch time comes from HIP events recorded on the stepper's own stream around
 the same region. ``cpu_baseline`` times the NumPy oracle (one core) on a bounded sample.
 ``batch`` and ``serve`` carry the solves/s and p99 half of the metric on a bounded sample of the
-mixed batch workload (``--batch-n``; see ``batch_sample``).
+mixed batch workload (``--batch-n``; see ``batch_sample``). ``comparators`` times the reference's
+own torch-eager GPU rung on the same device and field (``torch_eager_comparator``).
 """
 
 from __future__ import annotations
@@ -139,6 +140,86 @@ def cpu_baseline(n: int, min_time: float) -> dict:
                       f"({os.cpu_count()} cores visible)"}
 
 
+def torch_eager_child(n: int, steps: int, warmup: int) -> int:
+    """Child-process half of ``torch_eager_comparator``; never loads the HIP library.
+
+    PyTorch-ROCm brings its own HIP runtime, and a second runtime in a process that already
+    holds the device finds no GPU, so the comparator runs in a process of its own. Prints one
+    JSON line: time per step and a SHA-256 of the final float32 field.
+    """
+    import hashlib
+
+    import torch
+
+    from pyheat3d_amd import analytic
+    from pyheat3d_amd.problem import heat_stencil
+
+    u0 = analytic.sine_modes_periodic(n, 3)
+    stencil = heat_stencil(3, 0.15, "periodic")
+    c0, c1 = float(stencil.c0), float(stencil.c1)
+
+    def step(u, out):
+        # backends/torch_backends.py:24-56: one roll pair per axis (x first), one expression
+        s = (torch.roll(u, 1, 2) + torch.roll(u, -1, 2)) + (torch.roll(u, 1, 1) + torch.roll(u, -1, 1))
+        s = s + (torch.roll(u, 1, 0) + torch.roll(u, -1, 0))
+        out.copy_(c0 * u + c1 * s)
+
+    dev = torch.device("cuda")
+    a = torch.from_numpy(u0).to(dev)
+    b = torch.empty_like(a)
+    for _ in range(warmup):
+        step(a, b)
+        a, b = b, a
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step(a, b)
+        a, b = b, a
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    got = a.cpu().numpy()
+    print(json.dumps({"ms_per_step": ms, "sha256": hashlib.sha256(got.tobytes()).hexdigest()}))
+    return 0
+
+
+def torch_eager_comparator(u0: np.ndarray, stencil, steps: int, warmup: int) -> dict:
+    """The reference's GPU rung 1, ``torch-eager``, on this GPU (a same-device comparator).
+
+    Restates the periodic path of ``backends/torch_backends.py:24-56`` (``_pairs`` with
+    ``torch.roll`` per axis, x first, then the update written as one expression, each operator
+    its own kernel) on PyTorch-ROCm, in a child process (``torch_eager_child``), timed with torch
+    events over ``steps`` steps after ``warmup``. Its final field is compared bit for bit (by
+    SHA-256) with the z-march stepper run for the same number of steps from the same field.
+    (The reference's ``torch-compile`` rung generates Triton kernels and is not run here.)
+    """
+    import hashlib
+    import subprocess
+
+    from pyheat3d_amd.registry import make_stepper
+
+    n = u0.shape[0]
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--torch-child",
+                        str(n), str(steps), str(warmup)], capture_output=True, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip()[-400:]}
+    child = json.loads(r.stdout.strip().splitlines()[-1])
+    st = make_stepper("hip-cdna4/zmarch", u0, stencil)
+    try:
+        st.step(warmup + steps)
+        ours = st.get()
+    finally:
+        st.close()
+    cells = math.prod(u0.shape)
+    ms = child["ms_per_step"]
+    return {"what": "reference torch-eager rung (torch.roll per axis, one expression) on "
+                    "PyTorch-ROCm, same GPU and field, child process",
+            "glups": round(cells / ms / 1e6, 2), "ms_per_step": round(ms, 3), "steps": steps,
+            "bitwise_equal_to_zmarch": hashlib.sha256(ours.tobytes()).hexdigest() == child["sha256"]}
+
+
 def batch_sample(n: int) -> tuple[dict, dict]:
     """The solves/s + p99 half of the metric on a bounded sample of the batch mix.
 
@@ -187,9 +268,14 @@ def main() -> int:
     ap.add_argument("--cpu-min-time", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth probe")
+    ap.add_argument("--no-torch", action="store_true", help="skip the torch-eager comparator")
+    ap.add_argument("--torch-child", nargs=3, type=int, metavar=("N", "STEPS", "WARMUP"),
+                    help=argparse.SUPPRESS)
     ap.add_argument("--batch-n", type=int, default=512,
                     help="solves in the batched-vs-per-solve and served sample (0: skip)")
     a = ap.parse_args()
+    if a.torch_child:
+        return torch_eager_child(*a.torch_child)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the multi-GPU z-slab design is documented, not built)", file=sys.stderr)
@@ -233,6 +319,14 @@ def main() -> int:
                        "minimal_GBs": round(r["effective_GBs"], 1), "k": r["k"],
                        "single_step_equiv_GBs": round(r["glups"] * 8, 1),
                        "max_mode_amplitude_rel_err": max(r["check"]["mode_amplitude_rel_err"])}
+    torch_cmp = None
+    if not a.no_torch:
+        torch_cmp = torch_eager_comparator(u0, stencil, 20, 3)
+        if "error" in torch_cmp:
+            log(f"torch-eager comparator failed: {torch_cmp['error']}")
+        else:
+            log(f"torch-eager comparator: {torch_cmp['glups']:.1f} GLUPS, bitwise equal to "
+                f"zmarch: {torch_cmp['bitwise_equal_to_zmarch']}")
     del u0
 
     achieved = head["algorithmic_bytes_per_launch"] / (head["kernel_avg_ms"] * 1e-3) / 1e9
@@ -265,6 +359,7 @@ def main() -> int:
                   "what": "amplitude of each sine mode after the run vs the closed form g_m^steps"},
         "host_transfer": head["host_transfer"],
         "rungs": others,
+        "comparators": {"torch-eager": torch_cmp},
         "batch": batch,
         "serve": serve,
         "device": {"name": info["name"], "arch": info["arch"], "cu_count": info["cu_count"]},
