@@ -46,6 +46,13 @@ class Request:
     stall_fraction: float = 0.05
     source: np.ndarray | None = None   # optional float32 source (Poisson)
     rid: int = 0
+    bc: str = "dirichlet"         # "dirichlet" | "periodic" (heat only)
+
+    def __post_init__(self) -> None:
+        if self.bc not in ("dirichlet", "periodic"):
+            raise ValueError(f"bc must be 'dirichlet' or 'periodic', got {self.bc!r}")
+        if self.bc == "periodic" and self.mode == "jacobi":
+            raise ValueError("periodic Jacobi has no unique solution; use bc='dirichlet'")
 
     @property
     def dims(self) -> int:
@@ -63,7 +70,7 @@ class Request:
     def stencil(self):
         if self.mode == "heat":
             r = self.r if self.r is not None else (0.2 if self.dims == 2 else 0.15)
-            return heat_stencil(self.dims, r)
+            return heat_stencil(self.dims, r, self.bc)
         return jacobi_stencil(self.dims)
 
     def device_bytes(self) -> int:
@@ -72,7 +79,8 @@ class Request:
 
     def work(self) -> int:
         """Lattice updates if the solve runs to its cap (a bound for Jacobi)."""
-        return int(np.prod([s - 2 for s in self.shape])) * self.steps
+        inner = 0 if self.bc == "periodic" else 2
+        return int(np.prod([s - inner for s in self.shape])) * self.steps
 
 
 @dataclass
@@ -150,7 +158,7 @@ def make_desc(req: Request, f: DeviceFields) -> _hip.SolveDesc:
     d = _hip.SolveDesc()
     d.dims = req.dims
     d.nz, d.ny, d.nx = (1, *req.shape) if req.dims == 2 else req.shape
-    d.periodic = 0
+    d.periodic = 1 if req.bc == "periodic" else 0
     d.storage = _hip.STORAGE_CODE[req.storage]
     d.c0, d.c1 = float(np.float32(st.c0)), float(np.float32(st.c1))
     d.buf[0], d.buf[1] = f.buf

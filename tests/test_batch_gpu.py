@@ -77,6 +77,25 @@ def test_batch_matches_oracle(storage):
                 assert r.rel_residual <= req.tol
 
 
+def test_periodic_heat_matches_oracle():
+    """Periodic solves in the varlen passes (fused and single steps, 2D rows of odd and even
+    width, 3D), next to Dirichlet ones of the same shapes."""
+    shapes = [(200, 201), (190, 256), (300, 129), (40, 36, 30), (33, 34, 35)]
+    reqs = []
+    for i, sh in enumerate(shapes):
+        for bc in ("periodic", "dirichlet"):
+            for steps in (40, 37):
+                reqs.append(Request(shape=sh, mode="heat", steps=steps, seed=77 + 5 * i, bc=bc,
+                                    rid=len(reqs)))
+    res = run(reqs)
+    for req, r in zip(reqs, res):
+        u0 = splitmix_field(req.shape, req.seed, req.kind)
+        st = req.stencil()
+        assert r.status == "done" and r.steps == req.steps
+        ref = nr.emulated_reference(u0, st.c0, st.c1, req.bc, req.steps, "float32", 1)
+        assert np.array_equal(r.u, ref), (req.shape, req.bc, req.steps, nr.rel_err(r.u, ref))
+
+
 def test_alone_equals_in_mix():
     reqs = mix()
     together = run(reqs)

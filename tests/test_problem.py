@@ -144,6 +144,13 @@ def test_request_mix_is_deterministic_and_in_range():
     assert {r.mode for r in a} == {"heat", "jacobi"}
     r = Request(shape=(10, 12), mode="heat")
     assert r.stencil().c1 == 0.2 and r.device_bytes() == 2 * 120 * 4 and r.kind == 0
+    assert r.stencil().bc == "dirichlet" and r.work() == 8 * 10 * 200
+    p = Request(shape=(10, 12), mode="heat", bc="periodic")
+    assert p.stencil().bc == "periodic" and p.work() == 120 * 200
+    with pytest.raises(ValueError):
+        Request(shape=(10, 12), mode="jacobi", bc="periodic")
+    with pytest.raises(ValueError):
+        Request(shape=(10, 12), bc="neumann")
 
 
 def test_cli_list_and_json(capsys):
