@@ -390,6 +390,8 @@ struct ph3_batch {
     int n_init[3] = {0, 0, 0};
     std::vector<Tier2Class> classes;
     std::vector<void*> owned;                       // per-set device allocations
+    hipStream_t side = nullptr;                     // tier 1 runs here, beside tier 2
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static void free_set(ph3_batch* b) {
@@ -423,8 +425,14 @@ extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, sizeof(int));
     if (e == hipSuccess)
         e = hipMalloc((void**)&b->d_partial, sizeof(double) * ph3::kSlices * (size_t)capacity);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         (void)hipFree(b->d_desc); (void)hipFree(b->d_counter); (void)hipFree(b->d_partial);
+        if (b->side) (void)hipStreamDestroy(b->side);
+        if (b->fork) (void)hipEventDestroy(b->fork);
+        if (b->join) (void)hipEventDestroy(b->join);
         delete b;
         return ph3host::check(e, "ph3_batch_create");
     }
@@ -438,6 +446,9 @@ extern "C" int ph3_batch_destroy(ph3_batch* b) {
     (void)hipFree(b->d_desc);
     (void)hipFree(b->d_counter);
     (void)hipFree(b->d_partial);
+    (void)hipStreamDestroy(b->side);
+    (void)hipEventDestroy(b->fork);
+    (void)hipEventDestroy(b->join);
     delete b;
     return PH3_OK;
 }
@@ -569,19 +580,31 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const int segs = (sweeps + PH3_BATCH_CHECK - 1) / PH3_BATCH_CHECK;
     const int rs = segs * PH3_BATCH_CHECK;
+    // Tier 1 and tier 2 touch disjoint solves, so when both are present tier 1 runs on the
+    // batch's side stream, forked from and joined back to `stream` with events (in a captured
+    // graph: two parallel branches). Its few long-running workgroups then no longer keep the
+    // tier-2 passes waiting.
+    const bool any_small = b->n_small[0] + b->n_small[1] + b->n_small[2] > 0;
+    const bool fork = any_small && b->n_large > 0;
+    hipStream_t s1 = fork ? b->side : s;
+    if (fork) {
+        PH3_CHECK(hipEventRecord(b->fork, s));
+        PH3_CHECK(hipStreamWaitEvent(s1, b->fork, 0));
+    }
     // tier 1: one persistent launch per storage type present
     for (int st = 0; st < 3; ++st) {
         if (!b->n_small[st]) continue;
-        ph3::batch_reset<<<1, 1, 0, s>>>(b->d_counter);
+        ph3::batch_reset<<<1, 1, 0, s1>>>(b->d_counter);
         PH3_LAUNCH_CHECK("batch counter reset");
         const unsigned g = (unsigned)(b->n_small[st] < ph3::kPassGroups ? b->n_small[st] : ph3::kPassGroups);
         switch (st) {
-            case ph3::kF32: ph3::batch_small<float><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
-            case ph3::kF16: ph3::batch_small<_Float16><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
-            default: ph3::batch_small<ph3::bf16_t><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
+            case ph3::kF32: ph3::batch_small<float><<<g, ph3::kWG, 0, s1>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
+            case ph3::kF16: ph3::batch_small<_Float16><<<g, ph3::kWG, 0, s1>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
+            default: ph3::batch_small<ph3::bf16_t><<<g, ph3::kWG, 0, s1>>>(b->d_desc, b->d_small[st], b->n_small[st], b->d_counter, rs); break;
         }
         PH3_LAUNCH_CHECK("batch tier-1 launch");
     }
+    if (fork) PH3_CHECK(hipEventRecord(b->join, s1));
     // tier 2: segments of PH3_BATCH_CHECK varlen passes, then advance / residual / check
     if (b->n_large) {
         const unsigned gl = (unsigned)((b->n_large + 255) / 256);
@@ -615,6 +638,7 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
             PH3_LAUNCH_CHECK("batch check");
         }
     }
+    if (fork) PH3_CHECK(hipStreamWaitEvent(s, b->join, 0));
     return PH3_OK;
 }
 
