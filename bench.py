@@ -220,6 +220,55 @@ def torch_eager_comparator(u0: np.ndarray, stencil, steps: int, warmup: int) -> 
             "bitwise_equal_to_zmarch": hashlib.sha256(ours.tobytes()).hexdigest() == child["sha256"]}
 
 
+def _cpu_solve_seconds(args: tuple) -> float:
+    """One-core NumPy-oracle time of one batch request (module level, for a process pool).
+
+    Heat requests run all their steps. Jacobi requests time 20 sweeps and one FP64 residual and
+    extrapolate to the sweep count the GPU run reached, with a residual every 64 sweeps.
+    """
+    from oracle import numpy_ref as nr
+    from pyheat3d_amd.analytic import splitmix_field
+
+    shape, seed, kind, c0, c1, mode, sweeps = args
+    s = nr.NumpyStepper(splitmix_field(shape, seed, kind), c0, c1, "dirichlet")
+    t0 = time.perf_counter()
+    if mode == "heat":
+        s.step(sweeps)
+        return time.perf_counter() - t0
+    s.step(20)
+    t_sweep = (time.perf_counter() - t0) / 20
+    t0 = time.perf_counter()
+    nr.residual_fp64(s.get(), c1, "dirichlet")
+    t_res = time.perf_counter() - t0
+    return t_sweep * sweeps + t_res * (sweeps // 64 + 1)
+
+
+def cpu_batch_baseline(reqs: list, sweeps: dict[int, int]) -> dict:
+    """The NumPy oracle on the per-solve sample of the batch mix: one core, then every core."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+
+    jobs = []
+    for q in reqs:
+        st = q.stencil()
+        jobs.append((q.shape, q.seed, q.kind, st.c0, st.c1, q.mode, sweeps[q.rid]))
+    t0 = time.perf_counter()
+    one = [_cpu_solve_seconds(j) for j in jobs]
+    measured = time.perf_counter() - t0
+    cores = max(1, min(16, os.cpu_count() or 1))
+    # spawned workers: fresh interpreters that never touch the GPU this process holds
+    with ProcessPoolExecutor(cores, mp_context=multiprocessing.get_context("spawn")) as ex:
+        many = list(ex.map(_cpu_solve_seconds, jobs))
+    return {"kind": "port", "solves": len(jobs),
+            "solves_per_s_1core": round(len(jobs) / sum(one), 4),
+            "solves_per_s_all_cores": round(len(jobs) * cores / sum(many), 4),
+            "cores": cores,
+            "sample": "NumPy oracle (Dirichlet slicing stepper) on the per-solve sample; heat "
+                      "solves run in full, Jacobi solves timed over 20 sweeps + one FP64 residual "
+                      "and extrapolated to the GPU run's sweep count; all-core figure assumes "
+                      f"independent solves on {cores} processes ({measured:.1f} s measured)"}
+
+
 def batch_sample(n: int) -> tuple[dict, dict]:
     """The solves/s + p99 half of the metric on a bounded sample of the batch mix.
 
@@ -238,6 +287,12 @@ def batch_sample(n: int) -> tuple[dict, dict]:
         f"{b['per_solve']['solves_per_s']:.2f} per-solve ({b['speedup']:.1f}x) in "
         f"{time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
+    from pyheat3d_amd.batch import make_mix
+    sample = make_mix(n, 2024, n2=n2, n3=n3, frac_jacobi=0.5)[::16]
+    cpu = cpu_batch_baseline(sample, b["sample_sweeps"])
+    log(f"batch cpu baseline: {cpu['solves_per_s_1core']:.3f} solves/s on 1 core, "
+        f"{cpu['solves_per_s_all_cores']:.3f} on {cpu['cores']} in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
     s = serve_benchmark(n, rate=1000.0, seed=2024, n2=n2, n3=n3)
     log(f"serve: {s['solves_per_s']:.1f} solves/s, p50 {s['p50_ms']:.0f} ms, "
         f"p99 {s['p99_ms']:.0f} ms in {time.perf_counter() - t0:.1f} s")
@@ -247,7 +302,8 @@ def batch_sample(n: int) -> tuple[dict, dict]:
              "per_solve_sample": b["per_solve"]["sample"],
              "speedup": round(b["speedup"], 2),
              "jacobi": b["jacobi"],
-             "baseline_agrees_on_steps_and_status": b["baseline_agrees_on_steps_and_status"]}
+             "baseline_agrees_on_steps_and_status": b["baseline_agrees_on_steps_and_status"],
+             "cpu_baseline": cpu}
     serve = {"workload": s["workload"], "solves_per_s": round(s["solves_per_s"], 2),
              "p50_ms": round(s["p50_ms"], 1), "p99_ms": round(s["p99_ms"], 1),
              "statuses": s["statuses"]}

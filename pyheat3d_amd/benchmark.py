@@ -200,6 +200,7 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
                    "capped": sum(r.status == "max_sweeps" for r in jac),
                    "max_rel_residual": max((r.rel_residual or 0.0) for r in jac) if jac else None},
         "baseline_agrees_on_steps_and_status": f"{agree}/{len(sample)}",
+        "sample_sweeps": {int(b.rid): int(by_rid[b.rid].steps) for b in base},
     }
 
 
