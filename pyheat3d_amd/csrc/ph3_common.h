@@ -77,11 +77,6 @@ __device__ __forceinline__ float from_next_lane(float v) {  // lane i gets lane 
         __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
 }
 
-__device__ __forceinline__ int wrap_lo(int i, int n) { return i < 0 ? i + n : i; }
-__device__ __forceinline__ int wrap_hi(int i, int n) { return i >= n ? i - n : i; }
-__device__ __forceinline__ int wrapi(int i, int n) { i %= n; return i < 0 ? i + n : i; }
-__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
-
 }  // namespace ph3
 
 // Host-side error plumbing shared by every translation unit of the library.

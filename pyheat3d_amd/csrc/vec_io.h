@@ -59,9 +59,4 @@ __device__ __forceinline__ void unpack(const Packed<T, V>& raw, float (&out)[V])
     for (int e = 0; e < V; ++e) out[e] = Codec<T>::load(tmp[e]);
 }
 
-template <int V>
-__device__ __forceinline__ void load_vec_f32(const float* __restrict__ p, size_t i, float (&out)[V]) {
-    load_vec<float, V>(p, i, out);
-}
-
 }  // namespace ph3
