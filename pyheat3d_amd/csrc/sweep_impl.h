@@ -20,6 +20,38 @@ int launch_zmarch(const SweepArgs& a);
 int launch_tb(const SweepArgs& a, int k);
 bool tb_supported(int dims, int k);
 
+// Workgroups of `kernel` (`block` threads) resident on the current device at once.
+inline long resident_groups(const void* kernel, int block) {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1)
+        nb = 1;
+    return (long)cus * nb;
+}
+
+// March chunk (>= min_chunk planes) for `tiles` tiles over n_march planes. Workgroups run in
+// waves of `resident`; a wave of chunk ch costs about ch + fill planes (fill: the halo planes
+// a chunk loads or the pipeline it fills), so among the chunks giving at least 4 waves (fewer
+// measured slower: the last workgroups of a single wave finish unevenly) the one minimising
+// waves * (ch + fill) is taken. A fixed chunk leaves the last wave part-empty for most sizes
+// that are not powers of two (at 768^3 the 64-plane chunk filled 84% of the last wave).
+inline int fill_chunk(int n_march, long tiles, long resident, int min_chunk, int fill) {
+    constexpr long kMinWaves = 4;
+    int best = min_chunk < n_march ? min_chunk : n_march;
+    long best_cost = -1;
+    for (int ch = best; ch <= n_march; ++ch) {
+        const long chunks = (n_march + ch - 1) / ch;
+        const long waves = (tiles * chunks + resident - 1) / resident;
+        if (waves < kMinWaves) break;   // longer chunks only give fewer waves
+        const long cost = waves * (ch + fill);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = ch; }
+    }
+    return best;
+}
+
 }  // namespace ph3
 
 // Expands BODY once per (storage type T, periodic flag P, source flag S) combination.

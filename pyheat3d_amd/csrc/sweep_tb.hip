@@ -75,7 +75,15 @@ static int tb3_launch(const SweepArgs& a) {
     const int tiles_x = (a.nx + G::OX - 1) / G::OX;
     const int tiles_y = (a.ny + OY - 1) / OY;
     const long tiles = (long)tiles_x * tiles_y;
-    const int zch = tb_chunk(a.nz, tiles, K, a.tile.zch);
+    int zch = a.tile.zch;
+    if (zch <= 0) {
+        static long resident[16] = {};
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        long& res = resident[dev & 15];
+        if (!res) res = resident_groups((const void*)tb3d<T, P, S, V, R, W, K>, 64 * W);
+        zch = fill_chunk(a.nz, tiles, res, 16 * K, 2 * K);  // a chunk fills a 2k-plane pipeline
+    }
     const long groups = tiles * ((a.nz + zch - 1) / zch);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb3d: grid too large"); return PH3_EINVAL; }
     tb3d<T, P, S, V, R, W, K><<<(unsigned)groups, 64 * W, 0, a.stream>>>(

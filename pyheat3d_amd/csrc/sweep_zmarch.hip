@@ -70,7 +70,15 @@ static int zmarch_launch3(const SweepArgs& a) {
     const int tiles_x = (a.nx + 64 * V - 1) / (64 * V);
     const int tiles_y = (a.ny + R * W - 1) / (R * W);
     const long tiles = (long)tiles_x * tiles_y;
-    const int zch = a.tile.zch > 0 ? a.tile.zch : pick_chunk(a.nz, tiles, 64, 1024);
+    int zch = a.tile.zch;
+    if (zch <= 0) {
+        static long resident[16] = {};
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        long& res = resident[dev & 15];
+        if (!res) res = resident_groups((const void*)zmarch3d<T, P, S, V, R, W>, 64 * W);
+        zch = fill_chunk(a.nz, tiles, res, 32, 2);
+    }
     const int chunks = (a.nz + zch - 1) / zch;
     const long groups = tiles * chunks;
     if (groups > 0x7fffffffL) { ph3host::set_error("zmarch: grid too large"); return PH3_EINVAL; }

@@ -62,7 +62,9 @@ def candidates(kind: str, k: int, dims: int, nx: int, storage: str = "float32") 
     vecs = (1, 2, 4) if storage == "float32" else (1, 2, 4, 8)
     out = []
     if kind == "zmarch" and dims == 3:
-        for (r, w), v, z in itertools.product(ZMARCH_SHAPES, vecs, (0, 128, 256)):
+        # chunks of 32-256 planes: sizes that are not powers of two need a chunk whose
+        # workgroup count fills the last wave of workgroups
+        for (r, w), v, z in itertools.product(ZMARCH_SHAPES, vecs, (0, 32, 48, 64, 96, 128, 256)):
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, w, 0))
     elif kind == "zmarch":
@@ -70,7 +72,7 @@ def candidates(kind: str, k: int, dims: int, nx: int, storage: str = "float32") 
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, 4, 0))
     elif kind == "tb" and dims == 3:
-        for (v, r, w), z in itertools.product(TB3_SHAPES, (0, 128, 256)):
+        for (v, r, w), z in itertools.product(TB3_SHAPES, (0, 64, 96, 128, 192, 256)):
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, w, 0))
     elif kind == "tb":
