@@ -57,7 +57,7 @@ class HipStepper(Stepper):
             from ..tune import lookup
 
             self.tile_override = lookup(self.kind, self.steps_per_pass, self.dims, self.storage,
-                                        self.bc, self.shape)
+                                        self.bc, self.shape, source is not None)
         self.tile = {"kind": self.kind, "k": self.steps_per_pass,
                      **(self.tile_override.as_dict() if self.tile_override else {})}
 

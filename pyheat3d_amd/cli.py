@@ -131,7 +131,7 @@ def _cmd_tune(a: argparse.Namespace) -> int:
     entries = []
     for key in a.variants:
         dims = a.dims if a.dims in VARIANTS[key].dims else VARIANTS[key].dims[0]
-        e = tune(key, (a.n,) * dims, a.bc)
+        e = tune(key, (a.n,) * dims, a.bc, source=a.source)
         print(f"best {key}: {e['tile']} {e['glups']:.1f} GLUPS")
         entries.append(e)
     if a.write:
@@ -207,6 +207,7 @@ def main(argv: list[str] | None = None) -> int:
     t.add_argument("--dims", type=int, default=3)
     t.add_argument("--n", type=int, default=1024)
     t.add_argument("--bc", default="periodic", choices=("dirichlet", "periodic"))
+    t.add_argument("--source", action="store_true", help="tune with a float32 source term")
     t.add_argument("--write", action="store_true")
     t.set_defaults(fn=_cmd_tune)
 

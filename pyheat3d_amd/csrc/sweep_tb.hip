@@ -112,11 +112,13 @@ static int tb2_launch(const SweepArgs& a) {
 template <typename T, bool P, bool S, int K>
 static int tb3_shape(const SweepArgs& a) {
     // Defaults measured on MI355X at 1024^3 float32 (vec, rows, waves): k=2 (4, 2, 16)
-    // 981 GLUPS; k=3 (2, 4, 16) 1291 GLUPS; k=4 (1, 4, 16) 977 GLUPS (register-bound).
+    // 1100 GLUPS; k=3 (2, 4, 16) 1620 GLUPS; k=4 (1, 4, 16) (register-bound). With a source
+    // the k source planes held per lane push 16-wave shapes of k >= 3 and 2-4 site lanes past
+    // 128 registers (spills: 377 vs 698 GLUPS for k=3 at 768^3), so those take 8 waves.
     const int vmax = K == 2 ? 4 : (K == 3 ? 2 : 1);
     int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), vmax);
     int r = a.tile.ty ? a.tile.ty : (v == 4 ? 2 : 4);
-    int w = a.tile.waves ? a.tile.waves : 16;
+    int w = a.tile.waves ? a.tile.waves : (S && K >= 3 && v > 1 ? 8 : 16);
     if (a.nx % v) { ph3host::set_error("tb3d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
 #define PH3_TB3(V_, R_, W_) \
     if (v == V_ && r == R_ && w == W_) return tb3_launch<T, P, S, V_, R_, W_, K>(a)

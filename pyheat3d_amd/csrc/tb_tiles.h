@@ -82,6 +82,11 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     constexpr bool kDirect = sizeof(T) == 4;
     float pref[R][V];
     Packed<T, V> prefp[R];
+    // float32 source: at march step iz, bq[s-1] holds plane iz-s (used by stage s); the planes
+    // shift one slot per step and plane iz is prefetched into bnext, so each plane of b is read
+    // once per pass instead of once per stage
+    float bq[S ? K : 1][R][V];
+    float bnext[R][V];
 #pragma unroll
     for (int s = 0; s < K; ++s)
 #pragma unroll
@@ -90,12 +95,22 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             for (int r = 0; r < R; ++r)
 #pragma unroll
                 for (int e = 0; e < V; ++e) q[s][p][r][e] = 0.f;
+    if constexpr (S) {
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int e = 0; e < V; ++e) bq[s][r][e] = 0.f;
+    }
     {
         const size_t zo = (size_t)wrap_or_clamp(z0 - K, nz, P) * plane;
+        const size_t zb = (size_t)wrap_or_clamp(z0 - K - 1, nz, P) * plane;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
             else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
+            if constexpr (S) load_vec<float, V>(b, zb + roff[r] + lx, bnext[r]);
         }
     }
     int par = 0;   // LDS slot parity, flipped once per stage
@@ -120,6 +135,21 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                 if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
                 else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
             }
+        }
+        if constexpr (S) {
+#pragma unroll
+            for (int s = K - 1; s > 0; --s)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < V; ++e) bq[s][r][e] = bq[s - 1][r][e];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int e = 0; e < V; ++e) bq[0][r][e] = bnext[r][e];
+            const size_t zb = (size_t)wrap_or_clamp(iz, nz, P) * plane;
+#pragma unroll
+            for (int r = 0; r < R; ++r) load_vec<float, V>(b, zb + roff[r] + lx, bnext[r]);
         }
         float res[R][V];
 #pragma unroll
@@ -149,12 +179,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 #pragma unroll
                 for (int e = 0; e < V; ++e) { up[e] = ctr[0][e]; dn[e] = ctr[R - 1][e]; }
             }
-            float bsrc[R][V];
-            if (S) {
-                const size_t zo = (size_t)wrap_or_clamp(c, nz, P) * plane;
-#pragma unroll
-                for (int r = 0; r < R; ++r) load_vec<float, V>(b, zo + roff[r] + lx, bsrc[r]);
-            }
+            const float (&bsrc)[R][V] = bq[S ? s - 1 : 0];   // source of plane c
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const float fromL = from_prev_lane(ctr[r][V - 1]);
@@ -225,12 +250,20 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     constexpr bool kDirect = sizeof(T) == 4;   // 16-bit rows stay packed until used
     float pref[V];
     Packed<T, V> prefp;
+    float bq[S ? K : 1][V], bnext[V];   // source rows iy-1 .. iy-K, as in tb3d_tile
 #pragma unroll
     for (int s = 0; s < K; ++s)
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
             for (int e = 0; e < V; ++e) q[s][p][e] = 0.f;
+    if constexpr (S) {
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+#pragma unroll
+            for (int e = 0; e < V; ++e) bq[s][e] = 0.f;
+        load_vec<float, V>(b, (size_t)wrap_or_clamp(y0 - K - 1, ny, P) * nx + lx, bnext);
+    }
     if constexpr (kDirect) load_vec<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx, pref);
     else prefp = load_packed<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx);
     for (int ib = y0 - K; ib < y1 + K; ib += 3) {
@@ -249,6 +282,15 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             if constexpr (kDirect) load_vec<T, V>(u, ro, pref);
             else prefp = load_packed<T, V>(u, ro);
         }
+        if constexpr (S) {
+#pragma unroll
+            for (int s = K - 1; s > 0; --s)
+#pragma unroll
+                for (int e = 0; e < V; ++e) bq[s][e] = bq[s - 1][e];
+#pragma unroll
+            for (int e = 0; e < V; ++e) bq[0][e] = bnext[e];
+            load_vec<float, V>(b, (size_t)wrap_or_clamp(iy, ny, P) * nx + lx, bnext);
+        }
         float res[V];
 #pragma unroll
         for (int s = 1; s <= K; ++s) {
@@ -258,8 +300,7 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             const float (&ctr)[V] = q[s - 1][(k + 1) % 3];
             const float (&qp)[V] = q[s - 1][(k + 2) % 3];
             float (&dst)[V] = s < K ? q[s < K ? s : 0][(k + 2) % 3] : res;
-            float bsrc[V];
-            if (S) load_vec<float, V>(b, (size_t)wrap_or_clamp(c, ny, P) * nx + lx, bsrc);
+            const float (&bsrc)[V] = bq[S ? s - 1 : 0];   // source of row c
             const float fromL = from_prev_lane(ctr[V - 1]);
             const float fromR = from_next_lane(ctr[0]);
 #pragma unroll

@@ -33,8 +33,10 @@ def bucket(shape: tuple[int, ...]) -> int:
     return int(round(math.log2(max(1, int(np.prod(shape))))))
 
 
-def table_key(kind: str, k: int, dims: int, storage: str, bc: str, shape: tuple[int, ...]) -> str:
-    return f"{kind}/k{k}/{dims}d/{storage}/{bc}/b{bucket(shape)}"
+def table_key(kind: str, k: int, dims: int, storage: str, bc: str, shape: tuple[int, ...],
+              source: bool = False) -> str:
+    """Table key; a float32 source adds a register queue to tb, so sourced runs tune apart."""
+    return f"{kind}/k{k}/{dims}d/{storage}/{bc}{'/src' if source else ''}/b{bucket(shape)}"
 
 
 _table: dict[str, Any] | None = None
@@ -48,9 +50,9 @@ def load_table() -> dict[str, Any]:
 
 
 def lookup(kind: str, k: int, dims: int, storage: str, bc: str,
-           shape: tuple[int, ...]) -> _hip.Tile | None:
+           shape: tuple[int, ...], source: bool = False) -> _hip.Tile | None:
     """Tuned tile for this problem, or None (the library's defaults apply)."""
-    e = load_table().get(table_key(kind, k, dims, storage, bc, shape))
+    e = load_table().get(table_key(kind, k, dims, storage, bc, shape, source))
     if not e:
         return None
     t = e["tile"]
@@ -83,15 +85,17 @@ def candidates(kind: str, k: int, dims: int, nx: int, storage: str = "float32") 
 
 
 def tune(key: str, shape: tuple[int, ...], bc: str = "periodic", steps: int = 20,
-         verbose: bool = True) -> dict[str, Any]:
-    """Time every candidate tile of variant ``key`` on ``shape``; return the best entry."""
+         verbose: bool = True, source: bool = False) -> dict[str, Any]:
+    """Time every candidate tile of variant ``key`` on ``shape`` (with a float32 source when
+    ``source``); return the best entry."""
     from .problem import heat_stencil
     from .registry import VARIANTS, make_stepper
 
     var = VARIANTS[key]
     dims = len(shape)
     u0 = np.full(shape, 0.5, np.float32)
-    st = make_stepper(key, u0, heat_stencil(dims, 0.9 / (2 * dims), bc))
+    b = np.full(shape, 1e-5, np.float32) if source else None
+    st = make_stepper(key, u0, heat_stencil(dims, 0.9 / (2 * dims), bc), b)
     kind, k = st.kind, st.steps_per_pass
     ev0, ev1 = _hip.Event(), _hip.Event()
     best = None
@@ -121,7 +125,7 @@ def tune(key: str, shape: tuple[int, ...], bc: str = "periodic", steps: int = 20
         st.close()
     if best is None:
         raise RuntimeError(f"no candidate tile ran for {key} {shape}")
-    best["key"] = table_key(kind, k, dims, var.storage, bc, shape)
+    best["key"] = table_key(kind, k, dims, var.storage, bc, shape, source)
     return best
 
 

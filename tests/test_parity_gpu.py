@@ -183,27 +183,30 @@ def test_tb3d_every_tile_shape(k):
 @pytest.mark.parametrize("key", ["hip-cdna4/tb2", "hip-cdna4/tb3", "hip-cdna4/tb4",
                                  "hip-cdna4/tb8", "hip-cdna4/tb4-bf16"])
 def test_tb2d_lane_vectors(key):
-    """2D temporal blocking with every lane vector width (even and odd widths), both BCs."""
+    """2D temporal blocking with every lane vector width (even and odd widths), both BCs, with
+    and without a source."""
     rng = np.random.default_rng(30)
     k = VARIANTS[key].steps_per_pass
     storage = VARIANTS[key].storage
     for shape in ((37, 136), (50, 264), (19, 97)):
         u0 = rng.random(shape, dtype=np.float32)
+        b = rng.random(shape, dtype=np.float32) * np.float32(1e-3)
         for bc in ("dirichlet", "periodic"):
             st = heat_stencil(2, 0.2, bc)
-            ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 2 * k + 1, storage, k)
-            for vec in (1, 2, 4, 8):
-                if shape[1] % vec or (vec == 8 and storage == "float32"):
-                    continue
-                s = make_stepper(key, u0, st)
-                s.tile_override = _hip.Tile(0, 0, vec, 4, 0)
-                print(f"[gpu] {key} {shape} {bc} vec={vec}", flush=True)
-                try:
-                    s.step(2 * k + 1)
-                    u = s.get()
-                finally:
-                    s.close()
-                assert np.array_equal(u, ref), (key, shape, bc, vec, nr.rel_err(u, ref))
+            for src in (None, b):
+                ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 2 * k + 1, storage, k, src)
+                for vec in (1, 2, 4, 8):
+                    if shape[1] % vec or (vec == 8 and storage == "float32"):
+                        continue
+                    s = make_stepper(key, u0, st, src)
+                    s.tile_override = _hip.Tile(0, 0, vec, 4, 0)
+                    print(f"[gpu] {key} {shape} {bc} src={src is not None} vec={vec}", flush=True)
+                    try:
+                        s.step(2 * k + 1)
+                        u = s.get()
+                    finally:
+                        s.close()
+                    assert np.array_equal(u, ref), (key, shape, bc, vec, nr.rel_err(u, ref))
 
 
 @pytest.mark.parametrize("bc", ["periodic", "dirichlet"])

@@ -171,3 +171,5 @@ def test_tuning_table_keys_resolve():
     assert key in table
     t = tune.lookup("zmarch", 1, 3, "float32", "periodic", (1024, 1024, 1024))
     assert t is not None and t.vec in (1, 2, 4)
+    assert tune.table_key("tb", 3, 3, "float32", "dirichlet", (8, 8, 8), source=True) == \
+        "tb/k3/3d/float32/dirichlet/src/b9"
