@@ -173,3 +173,15 @@ def test_tuning_table_keys_resolve():
     assert t is not None and t.vec in (1, 2, 4)
     assert tune.table_key("tb", 3, 3, "float32", "dirichlet", (8, 8, 8), source=True) == \
         "tb/k3/3d/float32/dirichlet/src/b9"
+
+
+def test_bench_cpu_solve_seconds_runs_the_oracle():
+    """The batch CPU baseline's per-solve timer (heat in full, Jacobi extrapolated)."""
+    import bench
+    from pyheat3d_amd.batch import Request
+
+    for mode, sweeps in (("heat", 30), ("jacobi", 640)):
+        q = Request(shape=(12, 14), mode=mode, steps=sweeps, seed=3)
+        st = q.stencil()
+        t = bench._cpu_solve_seconds((q.shape, q.seed, q.kind, st.c0, st.c1, q.mode, sweeps))
+        assert 0.0 < t < 5.0
