@@ -51,6 +51,34 @@ def test_server_results_equal_alone(budget):
         assert r.seconds >= 0
 
 
+def test_served_heat_accuracy_vs_fp64():
+    """Served heat solves against the FP64 oracle (same initial field, float64 arithmetic):
+    float32 within the reference's cross-backend tolerance, bfloat16 within its storage error."""
+    from oracle import numpy_ref as nr
+    from pyheat3d_amd.analytic import splitmix_field
+
+    shapes = [(130, 260), (36, 36, 36), (300, 150), (50, 40, 30)]
+    reqs = [Request(shape=sh, mode="heat", steps=100, seed=300 + i, rid=2 * i + j,
+                    storage=st)
+            for i, sh in enumerate(shapes) for j, st in enumerate(("float32", "bfloat16"))]
+    srv = Server(10 ** 9, round_sweeps=64, keep_fields=True)
+    try:
+        res, _ = srv.serve(reqs, poisson_arrivals(len(reqs), rate=500.0, seed=4), time_limit=120)
+    finally:
+        srv.close()
+    worst = {"float32": 0.0, "bfloat16": 0.0}
+    for req, r in zip(reqs, res):
+        st = req.stencil()
+        c0, c1 = float(np.float32(st.c0)), float(np.float32(st.c1))  # the device's coefficients
+        ref = splitmix_field(req.shape, req.seed, req.kind).astype(np.float64)
+        for _ in range(req.steps):
+            ref = nr.reference_step(ref, c0, c1, "dirichlet")
+        worst[req.storage] = max(worst[req.storage], nr.rel_err(r.u, ref))
+    print(f"[gpu] served heat vs FP64 oracle, max relative error: {worst}", flush=True)
+    assert worst["float32"] <= nr.TOLERANCE["float32"]
+    assert worst["bfloat16"] <= 0.1
+
+
 def test_oversized_request_is_rejected():
     srv = Server(10 ** 5)
     try:
