@@ -109,19 +109,19 @@ static int ymarch_launch2(const SweepArgs& a) {
 template <typename T, bool P, bool S, int V>
 static int zmarch_shape3(const SweepArgs& a) {
     // Defaults measured on MI355X at 1024^3 (rows per wave, waves): float32 with 16-byte lanes
-    // (4, 8) 705 GLUPS; 16-bit storage with 16-byte lanes (4, 4) 1300 GLUPS; narrower lanes
-    // (1, 16) or (4, 4).
+    // (8, 8) 716 GLUPS (64-row tiles: half the y-halo rows of (4, 8), 702); 16-bit storage
+    // with 16-byte lanes (4, 4) 1300 GLUPS; narrower lanes (1, 16) or (4, 4).
     int dr = 4, dw = 4;
-    if (sizeof(T) == 4 && V == 4) { dr = 4; dw = 8; }
+    if (sizeof(T) == 4 && V == 4) { dr = 8; dw = 8; }
     else if ((sizeof(T) == 4 && V == 2) || (sizeof(T) == 2 && V == 4)) { dr = 1; dw = 16; }
     const int R = a.tile.ty ? a.tile.ty : dr;
     const int W = a.tile.waves ? a.tile.waves : dw;
 #define PH3_ZM(r, w) if (R == r && W == w) return zmarch_launch3<T, P, S, V, r, w>(a)
     PH3_ZM(1, 16); PH3_ZM(2, 8); PH3_ZM(2, 4); PH3_ZM(4, 4); PH3_ZM(4, 8); PH3_ZM(8, 2);
-    PH3_ZM(4, 2); PH3_ZM(8, 1);
+    PH3_ZM(4, 2); PH3_ZM(8, 1); PH3_ZM(8, 8);
 #undef PH3_ZM
     ph3host::set_error("zmarch3d: (rows %d, waves %d) not built; built: (1,16) (2,8) (2,4) "
-                       "(4,4) (4,8) (8,2) (4,2) (8,1)", R, W);
+                       "(4,4) (4,8) (8,2) (4,2) (8,1) (8,8)", R, W);
     return PH3_EINVAL;
 }
 

@@ -25,7 +25,7 @@ from . import _hip
 
 TABLE = Path(os.environ.get("PH3_TUNING", Path(__file__).resolve().parent / "tuning" / "gfx950.json"))
 
-ZMARCH_SHAPES = [(1, 16), (2, 8), (2, 4), (4, 4), (4, 8), (8, 2), (4, 2), (8, 1)]
+ZMARCH_SHAPES = [(1, 16), (2, 8), (2, 4), (4, 4), (4, 8), (8, 2), (4, 2), (8, 1), (8, 8)]
 TB3_SHAPES = [(2, 4, 16), (1, 8, 8), (2, 4, 8), (4, 2, 16), (1, 4, 16)]  # (vec, rows, waves)
 
 

@@ -123,7 +123,7 @@ def test_odd_widths_and_vector_paths(key):
             _check(key, _run(key, u0, st, 7), ref)
 
 
-ZMARCH_SHAPES = [(1, 16), (2, 8), (2, 4), (4, 4), (4, 8), (8, 2), (4, 2), (8, 1)]
+ZMARCH_SHAPES = [(1, 16), (2, 8), (2, 4), (4, 4), (4, 8), (8, 2), (4, 2), (8, 1), (8, 8)]
 
 
 @pytest.mark.parametrize("storage", ["float32", "bfloat16"])
