@@ -66,6 +66,8 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--copies", type=int, default=3)
+    ap.add_argument("--tile", default=None, metavar="TY,ZCH,VEC,WAVES,FLAGS",
+                    help="launch shape instead of the tuning table's")
     a = ap.parse_args()
     if a.summarize:
         import json
@@ -90,12 +92,14 @@ def main() -> None:
     shape = (a.n,) * a.dims
     st = make_stepper(a.variant, np.full(shape, 0.5, np.float32),
                       heat_stencil(a.dims, 0.15 if a.dims == 3 else 0.2, "periodic"))
+    if a.tile:
+        st.tile_override = _hip.Tile(*(int(v) for v in a.tile.split(",")))
     try:
         st.step(a.steps * st.steps_per_pass)
         st.synchronize()
     finally:
         st.close()
-    print(f"{a.variant}: {a.steps} launches on {shape}", flush=True)
+    print(f"{a.variant}: {a.steps} launches on {shape}, tile {st.tile}", flush=True)
 
 
 if __name__ == "__main__":
