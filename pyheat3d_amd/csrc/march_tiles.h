@@ -67,6 +67,7 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
     const bool useR = lane == 63 || x + V >= nx;
     const int hxl = P ? (xl == 0 ? nx - 1 : xl - 1) : max(xl - 1, 0);
     const int hxr = P ? (xl + V >= nx ? xl + V - nx : xl + V) : min(xl + V, nx - 1);
+    const bool fixL = x == 0, fixR = x + V == nx;   // lanes holding a fixed x column
     const size_t plane = (size_t)nx * ny;
     const bool top = wave == 0, bot = wave == W - 1;
 
@@ -203,14 +204,13 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
                     if (S) w[e] = w[e] + bs[cur][r][e];
                 }
                 if (xin && rstore[r]) {
-                    const size_t at = pz + roff[r] + x;
-                    if (P || (x > 0 && x + V < nx)) {
-                        store_out<T, V>(o, at, w, nt);
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < V; ++e)
-                            if (x + e > 0 && x + e < nx - 1) st(o, at + e, w[e]);
+                    // Dirichlet: the fixed sites x = 0 and x = nx - 1 keep their value, which
+                    // the whole-vector store writes back unchanged (no per-site branch)
+                    if (!P) {
+                        if (fixL) w[0] = cc[r][0];
+                        if (fixR) w[V - 1] = cc[r][V - 1];
                     }
+                    store_out<T, V>(o, pz + roff[r] + x, w, nt);
                 }
             }
         }
@@ -238,6 +238,7 @@ ymarch2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
     const bool useR = lane == 63 || x + V >= nx;
     const int hxl = P ? (xl == 0 ? nx - 1 : xl - 1) : max(xl - 1, 0);
     const int hxr = P ? (xl + V >= nx ? xl + V - nx : xl + V) : min(xl + V, nx - 1);
+    const bool fixL = x == 0, fixR = x + V == nx;
 
     auto ridx = [&](int y) -> size_t {
         int yy = y;
@@ -295,14 +296,11 @@ ymarch2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
                 if (S) w[e] = w[e] + bs[r][e];
             }
             if (xin && yr < y1) {
-                const size_t at = (size_t)yr * nx + x;
-                if (P || (x > 0 && x + V < nx)) {
-                    store_vec<T, V>(o, at, w);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < V; ++e)
-                        if (x + e > 0 && x + e < nx - 1) st(o, at + e, w[e]);
+                if (!P) {   // fixed x columns keep their value (written back unchanged)
+                    if (fixL) w[0] = win[r + 1][0];
+                    if (fixR) w[V - 1] = win[r + 1][V - 1];
                 }
+                store_vec<T, V>(o, (size_t)yr * nx + x, w);
             }
         }
         if (more) {

@@ -131,7 +131,9 @@ static int tb3_shape(const SweepArgs& a) {
 
 template <typename T, bool P, bool S, int K>
 static int tb2_shape(const SweepArgs& a) {
-    int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), 16 / (int)sizeof(T));
+    // Default lane width: the k float32 levels of a lane's ring grow with it (k = 8: 58 VGPRs
+    // at 2 sites, 114 at 4, i.e. half the resident waves), so k = 8 takes 2 sites, others 4.
+    int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), K >= 8 ? 2 : 4);
     if (a.nx % v) { ph3host::set_error("tb2d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
     const int w = a.tile.waves ? a.tile.waves : 4;
     if (w != 4) { ph3host::set_error("tb2d: waves %d not built (4)", w); return PH3_EINVAL; }

@@ -54,6 +54,9 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     size_t roff[R];
     uint32_t fixmask = 0;      // bit r*V+e: site (row r, element e) is a fixed Dirichlet site
     uint32_t wmask = 0;        // bit r*V+e: site is written (inside output region and grid)
+    // Lanes are whole aligned V-site vectors and the row conditions are uniform per row, so a
+    // row's V bits of wmask are all set or all clear. Fixed sites are written too: they keep
+    // their value through every stage, so the store writes back what the grid already holds.
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int gy = gy0 + r;
@@ -66,10 +69,12 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             const int col = lane * V + e;
             const bool fixed = !P && (x <= 0 || x >= nx - 1 || gy <= 0 || gy >= ny - 1);
             if (fixed) fixmask |= 1u << (r * V + e);
-            if (row_out && col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx && !fixed)
+            if (row_out && col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx)
                 wmask |= 1u << (r * V + e);
         }
     }
+    // Dirichlet: whether any lane of this wave holds a fixed site (uniform over the wave)
+    const bool wave_fix = !P && __any(fixmask != 0u);
 
     // Plane queues in three register slots per level: at march step i (i = iz - z0 + K) level
     // s holds planes c-1, c, c+1 in slots i, i+1, i+2 (mod 3). A new value (the prefetched
@@ -195,9 +200,17 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                     sum = sum + (qm[r][e] + qp[r][e]);
                     float v = update(c0, c1, ctr[r][e], sum);
                     if (S) v = v + bsrc[r][e];
-                    if (!P && (zfixed || ((fixmask >> (r * V + e)) & 1u))) v = ctr[r][e];
                     dst[r][e] = v;
                 }
+            }
+            // Dirichlet: fixed sites keep their value; only a fixed plane or a wave that holds
+            // a fixed site runs the selects
+            if (!P && (zfixed || wave_fix)) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < V; ++e)
+                        if (zfixed || ((fixmask >> (r * V + e)) & 1u)) dst[r][e] = ctr[r][e];
             }
         }
         const int c = iz - K;
@@ -207,14 +220,8 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 constexpr uint32_t full = (1u << V) - 1u;
-                const uint32_t m = (wmask >> (r * V)) & full;
-                if (m == full) {
+                if (((wmask >> (r * V)) & full) == full)
                     store_vec<T, V>(o, zo + roff[r] + gx, res[r]);
-                } else if (m) {
-#pragma unroll
-                    for (int e = 0; e < V; ++e)
-                        if ((m >> e) & 1u) st(o, zo + roff[r] + gx + e, res[r][e]);
-                }
             }
         }
     }
@@ -242,8 +249,10 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         const int col = lane * V + e;
         const bool fixed = !P && (x <= 0 || x >= nx - 1);
         if (fixed) fixmask |= 1u << e;
-        if (col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx && !fixed) wmask |= 1u << e;
+        // whole lanes, fixed sites included (written back unchanged), as in tb3d_tile
+        if (col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx) wmask |= 1u << e;
     }
+    const bool wave_fix = !P && __any(fixmask != 0u);
     // row queues rotated by slot as in tb3d_tile (slots i, i+1, i+2 mod 3 hold rows c-1, c,
     // c+1 of level s at march step i)
     float q[K][3][V];
@@ -311,21 +320,17 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                 sum = sum + (qm[e] + qp[e]);
                 float v = update(c0, c1, ctr[e], sum);
                 if (S) v = v + bsrc[e];
-                if (!P && (yfixed || ((fixmask >> e) & 1u))) v = ctr[e];
                 dst[e] = v;
+            }
+            if (!P && (yfixed || wave_fix)) {   // fixed sites keep their value
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (yfixed || ((fixmask >> e) & 1u)) dst[e] = ctr[e];
             }
         }
         const int c = iy - K;
         if (wmask && c >= y0 && c < y1 && (P || (c > 0 && c < ny - 1))) {
-            const size_t at = (size_t)c * nx + gx;
-            constexpr uint32_t full = (1u << V) - 1u;
-            if (wmask == full) {
-                store_vec<T, V>(o, at, res);
-            } else {
-#pragma unroll
-                for (int e = 0; e < V; ++e)
-                    if ((wmask >> e) & 1u) st(o, at + e, res[e]);
-            }
+            store_vec<T, V>(o, (size_t)c * nx + gx, res);   // wmask is all V bits or none
         }
     }
     }

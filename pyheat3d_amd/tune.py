@@ -70,7 +70,7 @@ def candidates(kind: str, k: int, dims: int, nx: int, storage: str = "float32") 
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, w, 0))
     elif kind == "zmarch":
-        for r, v, z in itertools.product((1, 2, 4, 8), vecs, (0, 256)):
+        for r, v, z in itertools.product((1, 2, 4, 8), vecs, (0, 32, 64, 128, 256)):
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, 4, 0))
     elif kind == "tb" and dims == 3:
@@ -78,7 +78,7 @@ def candidates(kind: str, k: int, dims: int, nx: int, storage: str = "float32") 
             if nx % v == 0:
                 out.append(_hip.Tile(r, z, v, w, 0))
     elif kind == "tb":
-        for v, z in itertools.product(vecs, (0, 128, 512)):
+        for v, z in itertools.product(vecs, (0, 64, 128, 256, 512)):
             if nx % v == 0:
                 out.append(_hip.Tile(0, z, v, 4, 0))
     return out

@@ -123,6 +123,29 @@ def test_odd_widths_and_vector_paths(key):
             _check(key, _run(key, u0, st, 7), ref)
 
 
+@pytest.mark.parametrize("storage", ["float32", "bfloat16"])
+def test_naive_every_lane_width(storage):
+    """1, 2 and 4 sites per thread (partial last tiles, both BCs) equal the oracle bit for bit."""
+    _need("hip-cdna4/naive", 3)
+    rng = np.random.default_rng(16)
+    key = "hip-cdna4/naive" + ("" if storage == "float32" else "-bf16")
+    for shape, bc in (((19, 37, 100), "periodic"), ((13, 70, 97), "dirichlet"),
+                      ((5, 131, 517), "periodic"), ((37, 300), "dirichlet"), ((29, 257), "periodic")):
+        u0 = rng.random(shape, dtype=np.float32)
+        st = heat_stencil(len(shape), 0.9 / (2 * len(shape)), bc)
+        ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 5, storage, 1)
+        for vec in (1, 2, 4):
+            s = make_stepper(key, u0, st)
+            s.tile_override = _hip.Tile(0, 0, vec, 0, 0)
+            print(f"[gpu] {key} {shape} {bc} vec={vec}", flush=True)
+            try:
+                s.step(5)
+                u = s.get()
+            finally:
+                s.close()
+            assert np.array_equal(u, ref), (shape, vec, nr.rel_err(u, ref))
+
+
 ZMARCH_SHAPES = [(1, 16), (2, 8), (2, 4), (4, 4), (4, 8), (8, 2), (4, 2), (8, 1), (8, 8)]
 
 
