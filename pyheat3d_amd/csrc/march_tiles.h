@@ -57,8 +57,10 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
     const int x = (tx * 64 + lane) * V;
     const int J0 = ty * W * R;          // first row of the workgroup tile
     const int j0 = J0 + wave * R;       // first row of this wave
-    int z0 = cz * zch, z1 = min(z0 + zch, nz);
-    if (!P) { z0 = max(z0, 1); z1 = min(z1, nz - 1); }
+    // Dirichlet marches every plane too (plane indices of loads clamped, planes 0 and nz - 1
+    // not stored): clamping the march range to 1 .. nz-2 instead doubled the registers the
+    // compiler kept live (262 vs 150 VGPRs for 4 rows x 4 waves), spilling the taller tiles.
+    const int z0 = cz * zch, z1 = min(z0 + zch, nz);
     if (z0 >= z1) return;               // uniform over the workgroup
 
     const bool xin = x < nx;
@@ -85,22 +87,31 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
     const size_t hdn = (size_t)(P ? (J0 + W * R) % ny : min(J0 + W * R, ny - 1)) * nx;
 
     auto pidx = [&](int z) -> size_t {
-        const int zz = P ? (z < 0 ? z + nz : (z >= nz ? z - nz : z)) : z;
+        const int zz = P ? (z < 0 ? z + nz : (z >= nz ? z - nz : z)) : min(max(z, 0), nz - 1);
         return (size_t)zz * plane;
     };
 
     // Plane queue in four register slots: at step k of an unrolled group of four planes, slot
     // k holds plane z-1, k+1 plane z, k+2 plane z+1 and k+3 is free (slots mod 4). Rotating
-    // slot roles instead of values removes the per-plane register moves; halos and source
-    // double-buffer the same way with the parity of k.
+    // slot roles instead of values removes the per-plane register moves; the source
+    // double-buffers the same way with the parity of k.
     // Loads run one plane ahead. Float32 rows land in the free slot directly. Everything that
     // needs a conversion (16-bit rows, halo values) stays in its stored format until the step
     // that uses it, so that no conversion right after a load makes the wave wait there.
     constexpr bool kDirect = sizeof(T) == 4;
     float q[4][R][V];
     Packed<T, V> ahead[R];       // 16-bit storage: rows of the next plane, still packed
-    T xhl[2][R], xhr[2][R];      // x halos of plane z (used by edge lanes only)
-    Packed<T, V> ryu[2], ryd[2]; // memory halo rows of plane z (first / last wave only)
+    // Halo timing. The x halo scalars and the rows above / below the tile sit in cache lines
+    // that the neighbouring tiles load as their own rows two planes ahead. kAhead2: halos are
+    // loaded in that same step (slot (z - z0) & 3 holds plane z's), so the lines are still in
+    // L2; 1024^3 float32, 4 rows x 8 waves: reads 1.108 -> 1.037x the minimal bytes, 703 ->
+    // 731 GLUPS. It costs 2R + 2 registers per lane, which spills 8-row float32 tiles and
+    // slowed 16-bit storage (bf16 1358 -> 1174 GLUPS), so those load halos one plane ahead
+    // (slot parity k & 1).
+    constexpr bool kAhead2 = kDirect && R <= 4;
+    constexpr int HS = kAhead2 ? 4 : 2;
+    T xhl[HS][R], xhr[HS][R];      // x halos (used by edge lanes only)
+    Packed<T, V> ryu[HS], ryd[HS]; // memory halo rows (first / last wave only)
     float bs[2][R][V];           // source of plane z
 
     {
@@ -113,11 +124,20 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
             else ahead[r] = load_packed<T, V>(u, p1 + roff[r] + xl);
             xhl[0][r] = useL ? u[p0 + roff[r] + hxl] : T{};
             xhr[0][r] = useR ? u[p0 + roff[r] + hxr] : T{};
+            if constexpr (kAhead2) {
+                xhl[1][r] = useL ? u[p1 + roff[r] + hxl] : T{};
+                xhr[1][r] = useR ? u[p1 + roff[r] + hxr] : T{};
+            }
             if (S) load_vec<float, V>(b, p0 + roff[r] + xl, bs[0][r]);
         }
-        ryu[0] = ryu[1] = ryd[0] = ryd[1] = Packed<T, V>{};
+#pragma unroll
+        for (int h = 0; h < HS; ++h) ryu[h] = ryd[h] = Packed<T, V>{};
         if (top) ryu[0] = load_packed<T, V>(u, p0 + hup + xl);
         if (bot) ryd[0] = load_packed<T, V>(u, p0 + hdn + xl);
+        if constexpr (kAhead2) {
+            if (top) ryu[1] = load_packed<T, V>(u, p1 + hup + xl);
+            if (bot) ryd[1] = load_packed<T, V>(u, p1 + hdn + xl);
+        }
     }
 
     for (int zb = z0; zb < z1; zb += 4) {
@@ -134,28 +154,31 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
             const float (&cc)[R][V] = q[(k + 1) & 3];
             const float (&cn)[R][V] = q[(k + 2) & 3];
             const int cur = k & 1, nxt = cur ^ 1;
+            // halo slots of plane z and of the plane whose halos this step loads
+            const int hc = kAhead2 ? (k & 3) : cur, hn = kAhead2 ? ((k + 2) & 3) : nxt;
 
-            // prefetch for plane z+1: its upper neighbour z+2, its halos and its source
+            // prefetch: plane z+2's rows, the halos of plane z+2 (kAhead2) or z+1, and the
+            // source of plane z+1
             if (z + 1 < z1) {
-                const size_t p1 = pidx(z + 1), p2 = pidx(z + 2);
+                const size_t p1 = pidx(z + 1), p2 = pidx(z + 2), ph = kAhead2 ? p2 : p1;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (kDirect) load_vec<T, V>(u, p2 + roff[r] + xl, q[(k + 3) & 3][r]);
                     else ahead[r] = load_packed<T, V>(u, p2 + roff[r] + xl);
-                    xhl[nxt][r] = useL ? u[p1 + roff[r] + hxl] : T{};
-                    xhr[nxt][r] = useR ? u[p1 + roff[r] + hxr] : T{};
+                    xhl[hn][r] = useL ? u[ph + roff[r] + hxl] : T{};
+                    xhr[hn][r] = useR ? u[ph + roff[r] + hxr] : T{};
                     if (S) load_vec<float, V>(b, p1 + roff[r] + xl, bs[nxt][r]);
                 }
-                if (top) ryu[nxt] = load_packed<T, V>(u, p1 + hup + xl);
-                if (bot) ryd[nxt] = load_packed<T, V>(u, p1 + hdn + xl);
+                if (top) ryu[hn] = load_packed<T, V>(u, ph + hup + xl);
+                if (bot) ryd[hn] = load_packed<T, V>(u, ph + hdn + xl);
             }
 
             // rows above / below the wave: memory halo rows for the first / last wave, else
             // the neighbouring waves' first / last rows of plane z exchanged through LDS
             // (LDS parity (z - z0) & 1 == k & 1 since groups start at multiples of four)
             float yu[V], yd[V];
-            unpack<T, V>(ryu[cur], yu);
-            unpack<T, V>(ryd[cur], yd);
+            unpack<T, V>(ryu[hc], yu);
+            unpack<T, V>(ryd[hc], yd);
             if (W > 1) {
 #pragma unroll
                 for (int e = 0; e < V; ++e) {
@@ -187,10 +210,11 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
             }
 
             const size_t pz = (size_t)z * plane;
+            const bool zin = P || (z > 0 && z < nz - 1);   // Dirichlet: fixed planes stay
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 float xm[V], xp[V];
-                x_neighbours<V>(cc[r], Codec<T>::load(xhl[cur][r]), Codec<T>::load(xhr[cur][r]),
+                x_neighbours<V>(cc[r], Codec<T>::load(xhl[hc][r]), Codec<T>::load(xhr[hc][r]),
                                 useL, useR, xm, xp);
                 float w[V];
 #pragma unroll
@@ -203,7 +227,7 @@ zmarch3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restric
                     w[e] = update(c0, c1, cc[r][e], s);
                     if (S) w[e] = w[e] + bs[cur][r][e];
                 }
-                if (xin && rstore[r]) {
+                if (xin && rstore[r] && zin) {
                     // Dirichlet: the fixed sites x = 0 and x = nx - 1 keep their value, which
                     // the whole-vector store writes back unchanged (no per-site branch)
                     if (!P) {

@@ -8,7 +8,9 @@
 // (z-1, z, z+1) sit in registers and plane z+2 is prefetched one iteration ahead.
 // Neighbours:
 //   x: inside the lane, else from lane l-1 / l+1 by a DPP wave shift, else (wave edges,
-//      grid edges) from a halo value loaded one plane ahead;
+//      grid edges) from a halo value loaded ahead (with the plane's own rows, two planes
+//      ahead, for float32 tiles of up to 4 rows: the neighbouring tile loads those cache
+//      lines in the same step; one plane ahead otherwise);
 //   y: rows of the same lane; across waves, each wave publishes its first and last row of
 //      plane z in a double-buffered LDS slot (one barrier per plane); only the rows just above
 //      and below the whole workgroup tile are read from memory (by the first / last wave);
@@ -109,10 +111,11 @@ static int ymarch_launch2(const SweepArgs& a) {
 template <typename T, bool P, bool S, int V>
 static int zmarch_shape3(const SweepArgs& a) {
     // Defaults measured on MI355X at 1024^3 (rows per wave, waves): float32 with 16-byte lanes
-    // (8, 8) 716 GLUPS (64-row tiles: half the y-halo rows of (4, 8), 702); 16-bit storage
-    // with 16-byte lanes (4, 4) 1300 GLUPS; narrower lanes (1, 16) or (4, 4).
+    // (4, 8) 731 GLUPS (halos loaded two planes ahead; (8, 8), which has no registers for
+    // that, 716); 16-bit storage with 16-byte lanes (4, 4) 1300 GLUPS; narrower lanes (1, 16)
+    // or (4, 4).
     int dr = 4, dw = 4;
-    if (sizeof(T) == 4 && V == 4) { dr = 8; dw = 8; }
+    if (sizeof(T) == 4 && V == 4) { dr = 4; dw = 8; }
     else if ((sizeof(T) == 4 && V == 2) || (sizeof(T) == 2 && V == 4)) { dr = 1; dw = 16; }
     const int R = a.tile.ty ? a.tile.ty : dr;
     const int W = a.tile.waves ? a.tile.waves : dw;
