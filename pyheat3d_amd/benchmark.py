@@ -46,6 +46,8 @@ def machine_info() -> dict[str, Any]:
         d = _hip.device_info(0)
         info.update(gpu=d["name"] or "MI355X", arch=d["arch"], cu_count=d["cu_count"],
                     hbm_bytes=d["total_mem_bytes"])
+        ver = Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / ".info" / "version"
+        info["driver"] = "ROCm " + ver.read_text().strip() if ver.exists() else None
     except Exception as e:  # no device
         info["gpu"] = f"unavailable: {e}"
     info["date"] = _dt.datetime.now().astimezone().isoformat(timespec="seconds")
@@ -122,7 +124,11 @@ def run(mode: str = "quick", out_dir: str | Path = "results", variants: list[str
     copy_bw = copy_bandwidth(1 << 30) if hip_available() else None
     result: dict[str, Any] = {
         "machine": machine_info(), "mode": mode,
-        "ceiling": {"measured_copy_bandwidth_GBs": copy_bw, "spec_bandwidth_GBs": traffic.SPEC_BW_GBS},
+        # the reference's figure script reads copy_bandwidth_GBs (its roofline lines) and
+        # measured_copy_bandwidth_GBs (figures/make_figures.py:306-310)
+        "ceiling": {"copy_bandwidth_GBs": copy_bw or traffic.COPY_BW_GBS,
+                    "measured_copy_bandwidth_GBs": copy_bw,
+                    "spec_bandwidth_GBs": traffic.SPEC_BW_GBS},
         "traffic_model": "minimal: 2*bytes_per_value/steps_per_pass per lattice-site update",
         "runs": [],
     }

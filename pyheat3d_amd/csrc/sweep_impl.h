@@ -32,13 +32,18 @@ inline long resident_groups(const void* kernel, int block) {
     return (long)cus * nb;
 }
 
-// March chunk (>= min_chunk planes) for `tiles` tiles over n_march planes. Workgroups run in
-// waves of `resident`; a wave of chunk ch costs about ch + fill planes (fill: the halo planes
-// a chunk loads or the pipeline it fills), so among the chunks giving at least 4 waves (fewer
-// measured slower: the last workgroups of a single wave finish unevenly) the one minimising
+// March chunk for `tiles` tiles over n_march planes. Workgroups run in waves of `resident`;
+// a wave of chunk ch costs about ch + fill planes (fill: the halo planes a chunk loads or the
+// pipeline it fills), so among the chunks >= min_chunk giving at least 4 waves (fewer measured
+// slower: the last workgroups of a single wave finish unevenly) the one minimising
 // waves * (ch + fill) is taken. A fixed chunk leaves the last wave part-empty for most sizes
 // that are not powers of two (at 768^3 the 64-plane chunk filled 84% of the last wave).
-inline int fill_chunk(int n_march, long tiles, long resident, int min_chunk, int fill) {
+// A grid too small for 4 waves even at min_chunk is latency-bound instead: its time is about
+// one workgroup's march, so it gets the shortest chunks (no shorter than small_chunk) whose
+// workgroups still fit in one wave, at most four per CU of the 256 (more only add chunk
+// overhead). 128^3 z-march: 16 workgroups of 32 planes before, 50 GLUPS; now 331.
+inline int fill_chunk(int n_march, long tiles, long resident, int min_chunk, int fill,
+                      int small_chunk) {
     constexpr long kMinWaves = 4;
     int best = min_chunk < n_march ? min_chunk : n_march;
     long best_cost = -1;
@@ -49,7 +54,21 @@ inline int fill_chunk(int n_march, long tiles, long resident, int min_chunk, int
         const long cost = waves * (ch + fill);
         if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = ch; }
     }
-    return best;
+    if (best_cost >= 0) return best;
+    const long target = resident < 1024 ? resident : 1024;
+    long chunks = target / tiles;                             // whole chunks that fit one wave
+    if (chunks < 1) chunks = 1;
+    long ch = (n_march + chunks - 1) / chunks;
+    if (ch < small_chunk) ch = small_chunk;
+    if (ch > best) ch = best;
+    return ch < 1 ? 1 : (int)ch;
+}
+
+// Widest lane vector <= v whose 64-lane wave still fits the row: a wave of 64*v sites over a
+// narrower row leaves lanes idle (nx = 64 with 4 sites per lane: 48 of 64).
+inline int fit_vec(int v, int nx) {
+    while (v > 1 && 64 * v > nx) v >>= 1;
+    return v;
 }
 
 }  // namespace ph3

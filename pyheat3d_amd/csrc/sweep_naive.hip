@@ -106,7 +106,7 @@ int launch_naive(const SweepArgs& a) {
     // 4-byte and 2 for 2-byte storage). Measured on MI355X (GLUPS, vec 1 / 2 / 4): float32 3D
     // 1024^3 326 / 420 / 461, 2D 16384^2 397 / 565 / 536; bfloat16 3D 396 / 503 / 593, 2D
     // 450 / 598 / 635. More independent loads in flight per lane is what pays.
-    int vec = a.tile.vec ? a.tile.vec : (a.storage == kF32 && a.dims == 2 ? 2 : 4);
+    int vec = a.tile.vec ? a.tile.vec : fit_vec(a.storage == kF32 && a.dims == 2 ? 2 : 4, a.nx);
     PH3_DISPATCH_TPS(a, {
         switch (vec) {
             case 1: return naive_launch<T, P, S, 1>(a);

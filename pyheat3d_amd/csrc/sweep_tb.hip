@@ -56,6 +56,25 @@ static int tb_pick_vec(const SweepArgs& a, int elem_bytes, int maxv) {
     return 1;
 }
 
+// Default lane width for a k-step tile on a row of nx sites, starting from the widest allowed
+// v: a wave writes 64*v - 2*KV sites, so the share of launched sites that are output sites,
+// nx / (tiles * 64 * v), can drop for wide lanes on narrow rows (nx = 128, k = 3: 2-site lanes
+// need two tiles of 124 output sites, 50%; 1-site lanes three of 58, 67%). Narrower lanes are
+// taken when they launch over 10% fewer idle sites.
+template <int K>
+static int tb_fit_vec(int v, int nx) {
+    auto share = [&](int w) {
+        const int kv = ((K + w - 1) / w) * w;
+        const int ox = 64 * w - 2 * kv;
+        const long tiles = (nx + ox - 1) / ox;
+        return (double)nx / ((double)tiles * 64.0 * w);
+    };
+    int best = v;
+    for (int w = v >> 1; w >= 1; w >>= 1)
+        if (share(w) > 1.1 * share(best)) best = w;
+    return best;
+}
+
 static int tb_chunk(int n, long tiles, int k, int forced) {
     if (forced > 0) return forced;
     // ~1024 workgroups, chunks of at least 16k planes so the 2k-plane pipeline fill is small
@@ -82,7 +101,7 @@ static int tb3_launch(const SweepArgs& a) {
         (void)hipGetDevice(&dev);
         long& res = resident[dev & 15];
         if (!res) res = resident_groups((const void*)tb3d<T, P, S, V, R, W, K>, 64 * W);
-        zch = fill_chunk(a.nz, tiles, res, 16 * K, 2 * K);  // a chunk fills a 2k-plane pipeline
+        zch = fill_chunk(a.nz, tiles, res, 16 * K, 2 * K, 2 * K);  // a chunk fills a 2k-plane pipeline
     }
     const long groups = tiles * ((a.nz + zch - 1) / zch);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb3d: grid too large"); return PH3_EINVAL; }
@@ -98,7 +117,17 @@ static int tb2_launch(const SweepArgs& a) {
     using G = TbGeom<V, K>;
     const int strips = (a.nx + G::OX - 1) / G::OX;
     const int tiles_x = (strips + W - 1) / W;
-    const int ych = tb_chunk(a.ny, tiles_x, K, a.tile.zch);
+    int ych = a.tile.zch;
+    if (ych <= 0) {
+        ych = tb_chunk(a.ny, tiles_x, K, 0);
+        static long resident[16] = {};
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        long& res = resident[dev & 15];
+        if (!res) res = resident_groups((const void*)tb2d<T, P, S, V, K, W>, 64 * W);
+        if ((long)tiles_x * ((a.ny + ych - 1) / ych) < res)   // fewer workgroups than fit
+            ych = fill_chunk(a.ny, tiles_x, res, ych, 2 * K, 2 * K);
+    }
     const long groups = (long)tiles_x * ((a.ny + ych - 1) / ych);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb2d: grid too large"); return PH3_EINVAL; }
     tb2d<T, P, S, V, K, W><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
@@ -116,7 +145,7 @@ static int tb3_shape(const SweepArgs& a) {
     // the k source planes held per lane push 16-wave shapes of k >= 3 and 2-4 site lanes past
     // 128 registers (spills: 377 vs 698 GLUPS for k=3 at 768^3), so those take 8 waves.
     const int vmax = K == 2 ? 4 : (K == 3 ? 2 : 1);
-    int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), vmax);
+    int v = a.tile.vec ? a.tile.vec : tb_fit_vec<K>(tb_pick_vec(a, (int)sizeof(T), vmax), a.nx);
     int r = a.tile.ty ? a.tile.ty : (v == 4 ? 2 : 4);
     int w = a.tile.waves ? a.tile.waves : (S && K >= 3 && v > 1 ? 8 : 16);
     if (a.nx % v) { ph3host::set_error("tb3d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
@@ -133,7 +162,7 @@ template <typename T, bool P, bool S, int K>
 static int tb2_shape(const SweepArgs& a) {
     // Default lane width: the k float32 levels of a lane's ring grow with it (k = 8: 58 VGPRs
     // at 2 sites, 114 at 4, i.e. half the resident waves), so k = 8 takes 2 sites, others 4.
-    int v = a.tile.vec ? a.tile.vec : tb_pick_vec(a, (int)sizeof(T), K >= 8 ? 2 : 4);
+    int v = a.tile.vec ? a.tile.vec : tb_fit_vec<K>(tb_pick_vec(a, (int)sizeof(T), K >= 8 ? 2 : 4), a.nx);
     if (a.nx % v) { ph3host::set_error("tb2d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
     const int w = a.tile.waves ? a.tile.waves : 4;
     if (w != 4) { ph3host::set_error("tb2d: waves %d not built (4)", w); return PH3_EINVAL; }

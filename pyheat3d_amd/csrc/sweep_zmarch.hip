@@ -79,7 +79,7 @@ static int zmarch_launch3(const SweepArgs& a) {
         (void)hipGetDevice(&dev);
         long& res = resident[dev & 15];
         if (!res) res = resident_groups((const void*)zmarch3d<T, P, S, V, R, W>, 64 * W);
-        zch = fill_chunk(a.nz, tiles, res, 32, 2);
+        zch = fill_chunk(a.nz, tiles, res, 32, 2, 4);
     }
     const int chunks = (a.nz + zch - 1) / zch;
     const long groups = tiles * chunks;
@@ -97,7 +97,17 @@ template <typename T, bool P, bool S, int V, int R, int W>
 static int ymarch_launch2(const SweepArgs& a) {
     const int strips = (a.nx + 64 * V - 1) / (64 * V);
     const int tiles_x = (strips + W - 1) / W;
-    const int ych = a.tile.zch > 0 ? a.tile.zch : pick_chunk(a.ny, tiles_x, 32, 2048);
+    int ych = a.tile.zch;
+    if (ych <= 0) {
+        ych = pick_chunk(a.ny, tiles_x, 32, 2048);
+        static long resident[16] = {};
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        long& res = resident[dev & 15];
+        if (!res) res = resident_groups((const void*)ymarch2d<T, P, S, V, R, W>, 64 * W);
+        if ((long)tiles_x * ((a.ny + ych - 1) / ych) < res)   // fewer workgroups than fit
+            ych = fill_chunk(a.ny, tiles_x, res, ych, 2, 4);
+    }
     const int chunks = (a.ny + ych - 1) / ych;
     const long groups = (long)tiles_x * chunks;
     const int remap = (groups % 8) == 0 ? 1 : 0;
@@ -152,7 +162,7 @@ int launch_zmarch(const SweepArgs& a) {
         return PH3_EINVAL;
     }
     PH3_DISPATCH_TPS(a, {
-        int v = a.tile.vec ? a.tile.vec : pick_vec(a, (int)sizeof(T));
+        int v = a.tile.vec ? a.tile.vec : fit_vec(pick_vec(a, (int)sizeof(T)), a.nx);
         if (v < 1 || (v & (v - 1)) || v * (int)sizeof(T) > 16 || a.nx % v) {
             ph3host::set_error("zmarch: vec %d unsupported for nx %d and this storage", v, a.nx);
             return PH3_EINVAL;

@@ -14,6 +14,10 @@ from __future__ import annotations
 
 #: MI355X HBM3E peak (8 stacks, 8 TB/s), in GB/s
 SPEC_BW_GBS = 8000.0
+#: MI355X device-copy bandwidth, GB/s: this library's copy kernel on 2 GiB (6.12-6.15 TB/s
+#: measured over several boxes). The stand-in when no measurement is at hand, like the
+#: reference's measured-once ``COPY_BW_GBS`` for its GPU (``traffic.py``).
+COPY_BW_GBS = 6140.0
 #: MI355X dense FP32 vector peak, TFLOP/s (the stencil never gets near it: ~1 FLOP/B)
 SPEC_FP32_TFLOPS = 157.3
 #: bytes per stored value by storage name
