@@ -101,7 +101,10 @@ static int tb3_launch(const SweepArgs& a) {
         (void)hipGetDevice(&dev);
         long& res = resident[dev & 15];
         if (!res) res = resident_groups((const void*)tb3d<T, P, S, V, R, W, K>, 64 * W);
-        zch = fill_chunk(a.nz, tiles, res, 16 * K, 2 * K, 2 * K);  // a chunk fills a 2k-plane pipeline
+        // a chunk fills a 2k-plane pipeline; grids too small to fill the device take chunks
+        // down to one plane (more workgroups, each with a shorter chain of dependent steps:
+        // 64^3 tb2 / tb3 / tb4 29 / 24 / 33 -> 44 / 39 / 56 GLUPS)
+        zch = fill_chunk(a.nz, tiles, res, 16 * K, 2 * K, 1);
     }
     const long groups = tiles * ((a.nz + zch - 1) / zch);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb3d: grid too large"); return PH3_EINVAL; }
@@ -126,7 +129,7 @@ static int tb2_launch(const SweepArgs& a) {
         long& res = resident[dev & 15];
         if (!res) res = resident_groups((const void*)tb2d<T, P, S, V, K, W>, 64 * W);
         if ((long)tiles_x * ((a.ny + ych - 1) / ych) < res)   // fewer workgroups than fit
-            ych = fill_chunk(a.ny, tiles_x, res, ych, 2 * K, 2 * K);
+            ych = fill_chunk(a.ny, tiles_x, res, ych, 2 * K, 2 * K);   // (1 row: 2048^2 tb4 -9%)
     }
     const long groups = (long)tiles_x * ((a.ny + ych - 1) / ych);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb2d: grid too large"); return PH3_EINVAL; }
