@@ -274,7 +274,8 @@ def batch_sample(n: int) -> tuple[dict, dict]:
 
     ``n`` mixed solves (60% 2D N in [64, 256], 40% 3D N in [32, 64], half 200 heat steps, half
     Jacobi Laplace to a 1e-6 relative FP64 residual, float32): all submitted at once, batched
-    graph rounds vs one launch per sweep per solve (every 16th solve), then the same mix served
+    graph rounds vs one launch per sweep per solve (every 16th solve; also with the stepper's
+    replayed graphs of 16 sweeps), then the same mix served
     at 1000 Poisson arrivals per second for latency percentiles. The full-size mixes are in
     ``python -m pyheat3d_amd batch|serve``.
     """
@@ -284,8 +285,9 @@ def batch_sample(n: int) -> tuple[dict, dict]:
     t0 = time.perf_counter()
     b = batch_benchmark(n, seed=2024, n2=n2, n3=n3, frac_jacobi=0.5, baseline_every=16)
     log(f"batch: {b['batched']['solves_per_s']:.1f} solves/s batched vs "
-        f"{b['per_solve']['solves_per_s']:.2f} per-solve ({b['speedup']:.1f}x) in "
-        f"{time.perf_counter() - t0:.1f} s")
+        f"{b['per_solve']['solves_per_s']:.2f} per-solve ({b['speedup']:.1f}x; "
+        f"{b['per_solve_graphs']['solves_per_s']:.2f} with replayed graphs, "
+        f"{b['speedup_vs_graphs']:.1f}x) in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     from pyheat3d_amd.batch import make_mix
     sample = make_mix(n, 2024, n2=n2, n3=n3, frac_jacobi=0.5)[::16]
@@ -301,6 +303,8 @@ def batch_sample(n: int) -> tuple[dict, dict]:
              "per_solve_solves_per_s": round(b["per_solve"]["solves_per_s"], 3),
              "per_solve_sample": b["per_solve"]["sample"],
              "speedup": round(b["speedup"], 2),
+             "per_solve_graphs_solves_per_s": round(b["per_solve_graphs"]["solves_per_s"], 3),
+             "speedup_vs_graphs": round(b["speedup_vs_graphs"], 2),
              "jacobi": b["jacobi"],
              "baseline_agrees_on_steps_and_status": b["baseline_agrees_on_steps_and_status"],
              "cpu_baseline": cpu}

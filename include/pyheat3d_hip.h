@@ -143,14 +143,15 @@ int ph3_graph_destroy(void* graph_exec);
  * tile shape, check points, reduction order) depends only on the solve itself, so a solve
  * gives bit-identical results alone or in any batch.
  *
- * Tier 1 (grids of at most PH3_BATCH_SMALL_SITES sites): one workgroup owns the grid and runs
- * its sweeps back to back with workgroup barriers; a persistent kernel hands grids to
- * workgroups through an atomic counter. Tier 2 (larger grids): each pass is one varlen launch
+ * Tier 1 (grids of at most PH3_BATCH_SMALL_SITES sites: 64^2, 16^3): one workgroup owns the
+ * grid and runs its sweeps back to back with workgroup barriers; a persistent kernel hands
+ * grids to workgroups through an atomic counter. Larger grids advance faster as tier-2 tiles
+ * (a 181^2 grid: ~40 us per sweep on one workgroup). Tier 2 (larger grids): each pass is one varlen launch
  * whose workgroups loop over the z-march tiles of every active large grid (a per-grid tile
  * table), followed at each check point by a deterministic per-grid residual.
  * ------------------------------------------------------------------------------------------ */
 #define PH3_BATCH_CHECK 64
-#define PH3_BATCH_SMALL_SITES 32768
+#define PH3_BATCH_SMALL_SITES 4096
 #define PH3_STALL_RING 32
 
 typedef struct ph3_solve_desc {

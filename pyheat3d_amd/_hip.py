@@ -82,7 +82,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
 }
 
 BATCH_CHECK = 64          # PH3_BATCH_CHECK
-BATCH_SMALL_SITES = 32768  # PH3_BATCH_SMALL_SITES
+BATCH_SMALL_SITES = 4096  # PH3_BATCH_SMALL_SITES
 STALL_RING = 32           # PH3_STALL_RING
 
 

@@ -299,8 +299,11 @@ class BatchSolver:
 
 
 def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
-                     check_every: int = _hip.BATCH_CHECK) -> tuple[list[Result], float]:
-    """Baseline: each request on its own stepper, one launch per sweep, one after another.
+                     check_every: int = _hip.BATCH_CHECK,
+                     graphs: bool = False) -> tuple[list[Result], float]:
+    """Baseline: each request on its own stepper, one after another; one launch per sweep (the
+    reference's stepping), or with ``graphs`` the stepper's replayed graphs of 16 sweeps for
+    grids of up to ``HipStepper.GRAPH_MAX_SITES`` sites.
 
     Jacobi requests use the same residual definition, check cadence, tolerance, stagnation test
     and cap as the batch (:func:`pyheat3d_amd.jacobi.solve`).
@@ -321,6 +324,8 @@ def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
         stream.close()
         t0 = time.perf_counter()
         st = cls(u0, req.stencil(), req.source)
+        if not graphs:
+            st.GRAPH_MAX_SITES = 0
         try:
             if req.mode == "heat":
                 st.step(req.steps)

@@ -173,7 +173,9 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
     """All requests submitted at t = 0: batched graph rounds vs one launch per sweep per solve.
 
     The per-solve baseline runs every ``baseline_every``-th request (a systematic sample of the
-    same mix); its solves/s is the sample size over the sum of its solve times.
+    same mix); its solves/s is the sample size over the sum of its solve times. The same sample
+    is also solved with the single-grid stepper's replayed graphs (``per_solve_graphs``), the
+    stronger per-solve path for small grids.
     """
     from .batch import BatchSolver, make_mix, solve_one_by_one
 
@@ -188,6 +190,8 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
     sample = reqs[::baseline_every]
     base, _ = solve_one_by_one(sample)
     base_time = sum(r.seconds for r in base)
+    based, _ = solve_one_by_one(sample, graphs=True)
+    based_time = sum(r.seconds for r in based)
     jac = [r for q, r in zip(reqs, res) if q.mode == "jacobi"]
     by_rid = {r.rid: r for r in res}
     agree = sum(1 for b in base if by_rid[b.rid].steps == b.steps and by_rid[b.rid].status == b.status)
@@ -200,7 +204,9 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
         "per_solve": {"sample": len(sample), "seconds": base_time,
                       "solves_per_s": len(sample) / base_time,
                       **_lat_stats(np.array([r.seconds for r in base]) * 1e3)},
+        "per_solve_graphs": {"seconds": based_time, "solves_per_s": len(sample) / based_time},
         "speedup": (n / timing["seconds"]) / (len(sample) / base_time),
+        "speedup_vs_graphs": (n / timing["seconds"]) / (len(sample) / based_time),
         "jacobi": {"converged": sum(r.status == "done" for r in jac),
                    "stagnated": sum(r.status == "stagnated" for r in jac),
                    "capped": sum(r.status == "max_sweeps" for r in jac),

@@ -46,33 +46,66 @@ __host__ __device__ inline int cur_buf(const ph3_solve_desc& d) {
 
 // ---- one workgroup: sweep and residual of a whole grid ---------------------------------------
 
+// One sweep of a whole grid (at most PH3_BATCH_SMALL_SITES sites, so offsets fit in 32 bits)
+// by one workgroup. Each wave takes kRowsAtOnce rows at a time and issues the loads of all of
+// them before using any, instead of waiting on each row's loads in turn (a chain of dependent
+// L2 round trips per sweep: 181^2 grids, when tier 1 still took them, ran 140 -> 188 solves/s
+// on the bench mix). Rows that are out of range or fixed (Dirichlet) load from row 0 and
+// store nothing.
+constexpr int kRowsAtOnce = 8;
+
 template <typename T>
 __device__ void wg_sweep(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
                          int dims, int nz, int ny, int nx, bool P, float c0, float c1) {
+    constexpr int U = kRowsAtOnce, NW = kWG / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const size_t plane = (size_t)nx * ny;
+    const int plane = nx * ny;
     const int rows = nz * ny;
-    for (int row = wave; row < rows; row += kWG / 64) {
-        const int k = row / ny, j = row - k * ny;
-        if (!P && (j == 0 || j == ny - 1 || (dims == 3 && (k == 0 || k == nz - 1)))) continue;
-        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-        const size_t base = (size_t)k * plane, r0 = base + (size_t)j * nx;
-        const size_t rm = base + (size_t)jm * nx, rp = base + (size_t)jp * nx;
-        size_t zm = 0, zp = 0;
-        if (dims == 3) {
-            const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
-            zm = (size_t)km * plane + (size_t)j * nx;
-            zp = (size_t)kp * plane + (size_t)j * nx;
+    for (int row0 = wave; row0 < rows; row0 += NW * U) {
+        int r0[U], rm[U], rp[U], zm[U], zp[U];
+        bool ok[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int row = row0 + q * NW;
+            const int k = row / ny, j = row - k * ny;
+            ok[q] = row < rows &&
+                    (P || !(j == 0 || j == ny - 1 || (dims == 3 && (k == 0 || k == nz - 1))));
+            if (!ok[q]) { r0[q] = rm[q] = rp[q] = zm[q] = zp[q] = 0; continue; }
+            const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+            const int base = k * plane;
+            r0[q] = base + j * nx;
+            rm[q] = base + jm * nx;
+            rp[q] = base + jp * nx;
+            zm[q] = zp[q] = 0;
+            if (dims == 3) {
+                const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+                zm[q] = km * plane + j * nx;
+                zp[q] = kp * plane + j * nx;
+            }
         }
         for (int i = lane; i < nx; i += 64) {
-            if (!P && (i == 0 || i == nx - 1)) continue;
+            const bool xin = P || (i > 0 && i < nx - 1);
             const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-            float s = ld(u, r0 + im) + ld(u, r0 + ip);
-            s = s + (ld(u, rm + i) + ld(u, rp + i));
-            if (dims == 3) s = s + (ld(u, zm + i) + ld(u, zp + i));
-            float w = update(c0, c1, ld(u, r0 + i), s);
-            if (b) w = w + b[r0 + i];
-            st(o, r0 + i, w);
+            float cv[U], xl[U], xr[U], yl[U], yr[U], zl[U], zr[U], bv[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                cv[q] = ld(u, r0[q] + i);
+                xl[q] = ld(u, r0[q] + im);
+                xr[q] = ld(u, r0[q] + ip);
+                yl[q] = ld(u, rm[q] + i);
+                yr[q] = ld(u, rp[q] + i);
+                if (dims == 3) { zl[q] = ld(u, zm[q] + i); zr[q] = ld(u, zp[q] + i); }
+                if (b) bv[q] = b[r0[q] + i];
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                float s = xl[q] + xr[q];
+                s = s + (yl[q] + yr[q]);
+                if (dims == 3) s = s + (zl[q] + zr[q]);
+                float w = update(c0, c1, cv[q], s);
+                if (b) w = w + bv[q];
+                if (ok[q] && xin) st(o, r0[q] + i, w);
+            }
         }
     }
 }
@@ -252,7 +285,7 @@ struct T2Geom {
     long long tiles;
 };
 
-__host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K) {
+__host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K, int few) {
     T2Geom g;
     if (dims == 3) {
         if (K <= 1) {
@@ -274,8 +307,10 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
     // A k-step chunk recomputes 2k extra rows or planes, so deep 2D passes (k = 8) take chunks
     // of 64-128 rows (measured on mixes of 2048 2D/3D solves: 7.6% faster with half of them
     // Jacobi, 2.6% with all heat, than 16-64); shallower passes keep 8-64, where longer chunks
-    // measured slower on large 3D heat grids.
-    const int lo = K >= 8 ? 8 * K : (K <= 1 ? 8 : (2 * K > 8 ? 2 * K : 8));
+    // measured slower on large 3D heat grids. `few`: the class has fewer tiles than a pass has
+    // workgroups, so a pass lasts as long as one tile's march and deep 2D passes take chunks
+    // from 2k rows (more, shorter tiles) instead.
+    const int lo = K >= 8 ? (few ? 2 * K : 8 * K) : (K <= 1 ? 8 : (2 * K > 8 ? 2 * K : 8));
     const int hi = K >= 8 ? 16 * K : kMaxChunk;
     if (ch < lo) ch = lo;
     if (ch > hi) ch = hi;
@@ -287,7 +322,7 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
 template <typename T, bool P, bool S, int DIMS, int K>
 __global__ void __launch_bounds__(kWG)
 batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
-           const long long* __restrict__ prefix, int n, int p) {
+           const long long* __restrict__ prefix, int n, int p, int few) {
     const long long total = prefix[n];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = n - 1;  // last i with prefix[i] <= t
@@ -301,7 +336,7 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
             const int cur = (int)((d.done / K + p) & 1);
             const T* u = (const T*)d.buf[cur];
             T* o = (T*)d.buf[cur ^ 1];
-            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K);
+            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K, few);
             const int ix = (int)(local % g.tx);
             const long long rest = local / g.tx;
             if (DIMS == 3) {
@@ -372,6 +407,7 @@ __global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list,
 
 struct Tier2Class {
     int storage, dims, periodic, src, kpass;
+    int few = 0;   // tier2_geom's short-chunk mode (set from the class's tile count)
     std::vector<int> idx;
     int* d_list = nullptr;
     long long* d_prefix = nullptr;
@@ -508,7 +544,7 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
                 k.src == (d.src != nullptr) && k.kpass == d.kpass) c = &k;
         if (!c) {
             b->classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr,
-                                            d.kpass, {}});
+                                            d.kpass, 0, {}});
             c = &b->classes.back();
         }
         c->idx.push_back(i);
@@ -522,10 +558,16 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     if (int r = upload(b, large, &b->d_large, s)) return r;
     b->n_large = (int)large.size();
     for (auto& c : b->classes) {
+        long long total = 0;
+        for (int i : c.idx) {
+            const ph3_solve_desc& d = table[i];
+            total += ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, 0).tiles;
+        }
+        c.few = total < ph3::kPassGroups ? 1 : 0;
         std::vector<long long> prefix(c.idx.size() + 1, 0);
         for (size_t k = 0; k < c.idx.size(); ++k) {
             const ph3_solve_desc& d = table[c.idx[k]];
-            prefix[k + 1] = prefix[k] + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass).tiles;
+            prefix[k + 1] = prefix[k] + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few).tiles;
         }
         if (int r = upload(b, c.idx, &c.d_list, s)) return r;
         if (int r = upload(b, prefix, &c.d_prefix, s)) return r;
@@ -548,7 +590,7 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
 template <typename T, bool P, bool S, int DIMS, int K>
 static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
     ph3::batch_pass<T, P, S, DIMS, K><<<(unsigned)ph3::kPassGroups, ph3::kWG, 0, s>>>(
-        b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p);
+        b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p, c.few);
 }
 
 template <typename T, bool P, bool S, int DIMS>

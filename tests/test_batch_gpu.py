@@ -1,6 +1,6 @@
 """Batched solves on the GPU: parity with the oracle, alone == in the mix, batch == one by one.
 
-The mix spans both tiers (workgroup-owned grids of <= 32768 sites and varlen tile passes),
+The mix spans both tiers (workgroup-owned grids of <= 4096 sites and varlen tile passes),
 2D and 3D, heat and Jacobi, float32 and bfloat16.
 """
 
