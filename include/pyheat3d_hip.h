@@ -191,9 +191,10 @@ int ph3_batch_destroy(ph3_batch* batch);
 int ph3_batch_set(ph3_batch* batch, const ph3_solve_desc* descs, int n, void* stream);
 /* Enqueue one round: every running solve advances by up to `sweeps` steps (rounded up to a
  * multiple of PH3_BATCH_CHECK for mode 1), stopping early when finished. Asynchronous and
- * capturable into a graph (the launch shapes depend only on the table set last). Tier-1 work
- * runs on a stream of the batch, forked from and joined back to `stream` with events, so all
- * of it is ordered on `stream` as seen by the caller. */
+ * capturable into a graph (the launch shapes depend only on the table set last). Tier-1 work,
+ * and the passes of every tier-2 class after the first, run on streams of the batch, forked
+ * from and joined back to `stream` with events, so all of it is ordered on `stream` as seen by
+ * the caller. */
 int ph3_batch_round(ph3_batch* batch, int sweeps, void* stream);
 /* Copy the solve table back to descs[0..n) (host). Synchronous on `stream`. */
 int ph3_batch_get(ph3_batch* batch, ph3_solve_desc* descs, int n, void* stream);

@@ -28,6 +28,7 @@ constexpr int kPassGroups = 2048; // workgroups of the persistent / looping laun
 constexpr int kSlices = 16;       // residual slices per tier-2 grid
 constexpr int kTier2Tiles = 512; // tier-2 tiles a large grid is spread over
 constexpr int kMaxChunk = 64;     // longest tier-2 march chunk of passes with k < 8
+constexpr int kClassStreams = 3;  // streams for tier-2 classes beside the caller's
 constexpr int kR3 = 4, kW3 = 4;   // tier-2 3D tile: 4 waves x 4 rows, 64 sites wide
 constexpr int kR2 = 4, kW2 = 4;   // tier-2 2D tile: 4 waves x 64 columns, 4 rows per step
 constexpr int kTbR3 = 8, kTbW3 = 4; // tier-2 3D temporal-blocking tile: 4 waves x 8 rows
@@ -428,6 +429,9 @@ struct ph3_batch {
     std::vector<void*> owned;                       // per-set device allocations
     hipStream_t side = nullptr;                     // tier 1 runs here, beside tier 2
     hipEvent_t fork = nullptr, join = nullptr;
+    // tier-2 classes after the first run their passes on these, beside each other
+    hipStream_t cls[ph3::kClassStreams] = {};
+    hipEvent_t seg = nullptr, cls_join[ph3::kClassStreams] = {};
 };
 
 static void free_set(ph3_batch* b) {
@@ -464,11 +468,21 @@ extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->seg, hipEventDisableTiming);
+    for (int i = 0; i < ph3::kClassStreams && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&b->cls[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->cls_join[i], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         (void)hipFree(b->d_desc); (void)hipFree(b->d_counter); (void)hipFree(b->d_partial);
         if (b->side) (void)hipStreamDestroy(b->side);
         if (b->fork) (void)hipEventDestroy(b->fork);
         if (b->join) (void)hipEventDestroy(b->join);
+        if (b->seg) (void)hipEventDestroy(b->seg);
+        for (int i = 0; i < ph3::kClassStreams; ++i) {
+            if (b->cls[i]) (void)hipStreamDestroy(b->cls[i]);
+            if (b->cls_join[i]) (void)hipEventDestroy(b->cls_join[i]);
+        }
         delete b;
         return ph3host::check(e, "ph3_batch_create");
     }
@@ -485,6 +499,11 @@ extern "C" int ph3_batch_destroy(ph3_batch* b) {
     (void)hipStreamDestroy(b->side);
     (void)hipEventDestroy(b->fork);
     (void)hipEventDestroy(b->join);
+    (void)hipEventDestroy(b->seg);
+    for (int i = 0; i < ph3::kClassStreams; ++i) {
+        (void)hipStreamDestroy(b->cls[i]);
+        (void)hipEventDestroy(b->cls_join[i]);
+    }
     delete b;
     return PH3_OK;
 }
@@ -651,17 +670,31 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
     if (b->n_large) {
         const unsigned gl = (unsigned)((b->n_large + 255) / 256);
         const unsigned gr = (unsigned)(b->n_large * ph3::kSlices < ph3::kPassGroups ? b->n_large * ph3::kSlices : ph3::kPassGroups);
+        // Classes touch disjoint solves, so within a segment each runs its chain of passes on
+        // its own stream (the first on `stream`), forked and joined with events: a round then
+        // takes about the longest chain instead of the sum of all of them.
+        const int nc = (int)b->classes.size();
+        const int nside = nc - 1 < ph3::kClassStreams ? nc - 1 : ph3::kClassStreams;
         for (int seg = 0; seg < segs; ++seg) {
-            for (int p = 0; p < PH3_BATCH_CHECK; ++p) {
-                for (const auto& c : b->classes) {
-                    if (p >= PH3_BATCH_CHECK / c.kpass) continue;  // k-step passes: fewer per segment
+            if (nside > 0) {
+                PH3_CHECK(hipEventRecord(b->seg, s));
+                for (int i = 0; i < nside; ++i) PH3_CHECK(hipStreamWaitEvent(b->cls[i], b->seg, 0));
+            }
+            for (int ci = 0; ci < nc; ++ci) {
+                const auto& c = b->classes[ci];
+                hipStream_t sc = ci == 0 ? s : b->cls[(ci - 1) % nside];
+                for (int p = 0; p < PH3_BATCH_CHECK / c.kpass; ++p) {  // k-step passes: fewer
                     switch (c.storage) {
-                        case ph3::kF32: pass_class<float>(c, b, p, s); break;
-                        case ph3::kF16: pass_class<_Float16>(c, b, p, s); break;
-                        default: pass_class<ph3::bf16_t>(c, b, p, s); break;
+                        case ph3::kF32: pass_class<float>(c, b, p, sc); break;
+                        case ph3::kF16: pass_class<_Float16>(c, b, p, sc); break;
+                        default: pass_class<ph3::bf16_t>(c, b, p, sc); break;
                     }
                     PH3_LAUNCH_CHECK("batch tier-2 pass");
                 }
+            }
+            for (int i = 0; i < nside; ++i) {
+                PH3_CHECK(hipEventRecord(b->cls_join[i], b->cls[i]));
+                PH3_CHECK(hipStreamWaitEvent(s, b->cls_join[i], 0));
             }
             ph3::batch_advance<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->n_large);
             PH3_LAUNCH_CHECK("batch advance");
