@@ -532,8 +532,11 @@ static int check_desc(const ph3_solve_desc& d, int i) {
 static int choose_kpass(const ph3_solve_desc& d) {
     const long long sites = (long long)d.nz * d.ny * d.nx;
     if (sites <= PH3_BATCH_SMALL_SITES || d.storage != ph3::kF32) return 1;
-    int k = d.kpass > 1 ? d.kpass : (d.dims == 3 ? 2 : 8);
-    if (d.dims == 3 && k > 2) k = 2;
+    // 3D: 4-step passes from 128^3 sites (full-size heat mix 3012 -> 3714 solves/s), 2-step
+    // below (smaller grids pay more for the deeper halo: Jacobi mixes 5% slower with 4)
+    const int k3 = sites >= (1LL << 21) ? 4 : 2;
+    int k = d.kpass > 1 ? d.kpass : (d.dims == 3 ? k3 : 8);
+    if (d.dims == 3 && k > 4) k = 4;
     if (k > 8) k = 8;
     while (k > 1 && (PH3_BATCH_CHECK % k || d.steps % k)) k >>= 1;
     return k;
@@ -616,6 +619,9 @@ template <typename T, bool P, bool S, int DIMS>
 static void pass_k(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
     if constexpr (std::is_same<T, float>::value) {
         if (c.kpass == 2) return launch_pass<T, P, S, DIMS, 2>(c, b, p, s);
+        if constexpr (DIMS == 3) {
+            if (c.kpass == 4) return launch_pass<T, P, S, DIMS, 4>(c, b, p, s);
+        }
         if constexpr (DIMS == 2) {
             if (c.kpass == 4) return launch_pass<T, P, S, DIMS, 4>(c, b, p, s);
             if (c.kpass == 8) return launch_pass<T, P, S, DIMS, 8>(c, b, p, s);
