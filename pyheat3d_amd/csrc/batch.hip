@@ -286,14 +286,17 @@ struct T2Geom {
     long long tiles;
 };
 
-__host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K, int few) {
+__host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K, int few,
+                                             int V) {
     T2Geom g;
     if (dims == 3) {
         if (K <= 1) {
             g.tx = (nx + 63) / 64;
             g.ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
         } else {
-            g.tx = (nx + 64 - 2 * K - 1) / (64 - 2 * K);
+            // a tile of 64*V columns writes 64*V - 2*KV of them (TbGeom)
+            const int kv = ((K + V - 1) / V) * V, ox = 64 * V - 2 * kv;
+            g.tx = (nx + ox - 1) / ox;
             g.ty = (ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
         }
     } else {
@@ -320,10 +323,13 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
     return g;
 }
 
-template <typename T, bool P, bool S, int DIMS, int K>
+// V: sites per lane; 2 only for the class of float32 3D 2-step grids of even width (see
+// ph3_batch_set), 1 otherwise.
+template <typename T, bool P, bool S, int DIMS, int K, int V>
 __global__ void __launch_bounds__(kWG)
 batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
            const long long* __restrict__ prefix, int n, int p, int few) {
+    static_assert(V == 1 || (DIMS == 3 && K == 2), "2-site lanes only for 3D 2-step passes");
     const long long total = prefix[n];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = n - 1;  // last i with prefix[i] <= t
@@ -337,7 +343,7 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
             const int cur = (int)((d.done / K + p) & 1);
             const T* u = (const T*)d.buf[cur];
             T* o = (T*)d.buf[cur ^ 1];
-            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K, few);
+            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K, few, V);
             const int ix = (int)(local % g.tx);
             const long long rest = local / g.tx;
             if (DIMS == 3) {
@@ -346,7 +352,7 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
                     zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
                                                         d.c1, g.chunk, ix, iy, iz, 0);
                 else
-                    tb3d_tile<T, P, S, 1, kTbR3, kTbW3, K>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
+                    tb3d_tile<T, P, S, V, kTbR3, kTbW3, K>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
                                                            d.c1, g.chunk, ix, iy, iz);
             } else {
                 if constexpr (K == 1)
@@ -408,6 +414,7 @@ __global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list,
 
 struct Tier2Class {
     int storage, dims, periodic, src, kpass;
+    int vec = 1;   // sites per lane of the class's passes
     int few = 0;   // tier2_geom's short-chunk mode (set from the class's tile count)
     std::vector<int> idx;
     int* d_list = nullptr;
@@ -560,13 +567,19 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         const long long sites = (long long)d.nz * d.ny * d.nx;
         if (sites <= PH3_BATCH_SMALL_SITES) { small[d.storage].push_back(i); continue; }
         large.push_back(i);
+        // 2-site lanes for float32 3D 2-step grids of even width above 64 (a 128-wide tile
+        // then idles no more lanes than two 64-wide ones) with 8-byte aligned buffers
+        auto al8 = [](const void* q) { return ((uintptr_t)q & 7u) == 0; };
+        const int vec = d.dims == 3 && d.kpass == 2 && d.storage == ph3::kF32 && d.nx > 64 &&
+                        d.nx % 2 == 0 && al8(d.buf[0]) && al8(d.buf[1]) &&
+                        (!d.src || al8(d.src)) ? 2 : 1;
         Tier2Class* c = nullptr;
         for (auto& k : b->classes)
             if (k.storage == d.storage && k.dims == d.dims && k.periodic == (d.periodic != 0) &&
-                k.src == (d.src != nullptr) && k.kpass == d.kpass) c = &k;
+                k.src == (d.src != nullptr) && k.kpass == d.kpass && k.vec == vec) c = &k;
         if (!c) {
             b->classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr,
-                                            d.kpass, 0, {}});
+                                            d.kpass, vec, 0, {}});
             c = &b->classes.back();
         }
         c->idx.push_back(i);
@@ -583,13 +596,13 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         long long total = 0;
         for (int i : c.idx) {
             const ph3_solve_desc& d = table[i];
-            total += ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, 0).tiles;
+            total += ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, 0, c.vec).tiles;
         }
         c.few = total < ph3::kPassGroups ? 1 : 0;
         std::vector<long long> prefix(c.idx.size() + 1, 0);
         for (size_t k = 0; k < c.idx.size(); ++k) {
             const ph3_solve_desc& d = table[c.idx[k]];
-            prefix[k + 1] = prefix[k] + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few).tiles;
+            prefix[k + 1] = prefix[k] + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, c.vec).tiles;
         }
         if (int r = upload(b, c.idx, &c.d_list, s)) return r;
         if (int r = upload(b, prefix, &c.d_prefix, s)) return r;
@@ -609,16 +622,21 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     return PH3_OK;
 }
 
-template <typename T, bool P, bool S, int DIMS, int K>
+template <typename T, bool P, bool S, int DIMS, int K, int V = 1>
 static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
-    ph3::batch_pass<T, P, S, DIMS, K><<<(unsigned)ph3::kPassGroups, ph3::kWG, 0, s>>>(
+    ph3::batch_pass<T, P, S, DIMS, K, V><<<(unsigned)ph3::kPassGroups, ph3::kWG, 0, s>>>(
         b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p, c.few);
 }
 
 template <typename T, bool P, bool S, int DIMS>
 static void pass_k(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
     if constexpr (std::is_same<T, float>::value) {
-        if (c.kpass == 2) return launch_pass<T, P, S, DIMS, 2>(c, b, p, s);
+        if (c.kpass == 2) {
+            if constexpr (DIMS == 3) {
+                if (c.vec == 2) return launch_pass<T, P, S, DIMS, 2, 2>(c, b, p, s);
+            }
+            return launch_pass<T, P, S, DIMS, 2>(c, b, p, s);
+        }
         if constexpr (DIMS == 3) {
             if (c.kpass == 4) return launch_pass<T, P, S, DIMS, 4>(c, b, p, s);
         }
