@@ -26,6 +26,10 @@ def mix(storage="float32"):
             reqs.append(Request(shape=sh, mode=mode, storage=storage,
                                 steps=heat_steps if mode == "heat" else 384, seed=1000 + 17 * i,
                                 rid=len(reqs)))
+    # float32 3D grids of >= 128^3 sites fuse 4 steps per pass; (9, 70, 70) Jacobi above takes
+    # the 2-site-lane class of 3D 2-step grids of even width
+    reqs.append(Request(shape=(130, 128, 128), mode="heat", storage=storage, steps=40, seed=999,
+                        rid=len(reqs)))
     return reqs
 
 
@@ -79,8 +83,8 @@ def test_batch_matches_oracle(storage):
 
 def test_periodic_heat_matches_oracle():
     """Periodic solves in the varlen passes (fused and single steps, 2D rows of odd and even
-    width, 3D), next to Dirichlet ones of the same shapes."""
-    shapes = [(200, 201), (190, 256), (300, 129), (40, 36, 30), (33, 34, 35)]
+    width, 3D with 1- and 2-site lanes), next to Dirichlet ones of the same shapes."""
+    shapes = [(200, 201), (190, 256), (300, 129), (40, 36, 30), (33, 34, 35), (20, 24, 100)]
     reqs = []
     for i, sh in enumerate(shapes):
         for bc in ("periodic", "dirichlet"):
