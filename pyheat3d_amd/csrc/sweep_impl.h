@@ -3,6 +3,8 @@
 #include "ph3_common.h"
 #include "../../include/pyheat3d_hip.h"
 
+#include <atomic>
+
 namespace ph3 {
 
 struct SweepArgs {
@@ -30,6 +32,23 @@ inline long resident_groups(const void* kernel, int block) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1)
         nb = 1;
     return (long)cus * nb;
+}
+
+// resident_groups of one kernel, queried once per device (slot = device id mod 16; a stale
+// slot after a device change only mis-sizes chunks, never results). Keyed by the kernel
+// itself, so kernels sharing a signature do not share a cache.
+template <auto Kernel>
+inline long cached_resident(int block) {
+    static std::atomic<long> cache[16];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::atomic<long>& slot = cache[dev & 15];
+    long r = slot.load(std::memory_order_relaxed);
+    if (!r) {
+        r = resident_groups((const void*)Kernel, block);
+        slot.store(r, std::memory_order_relaxed);
+    }
+    return r;
 }
 
 // March chunk for `tiles` tiles over n_march planes. Workgroups run in waves of `resident`;

@@ -96,11 +96,7 @@ static int tb3_launch(const SweepArgs& a) {
     const long tiles = (long)tiles_x * tiles_y;
     int zch = a.tile.zch;
     if (zch <= 0) {
-        static long resident[16] = {};
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        long& res = resident[dev & 15];
-        if (!res) res = resident_groups((const void*)tb3d<T, P, S, V, R, W, K>, 64 * W);
+        const long res = cached_resident<tb3d<T, P, S, V, R, W, K>>(64 * W);
         // a chunk fills a 2k-plane pipeline; grids too small to fill the device take chunks
         // down to one plane (more workgroups, each with a shorter chain of dependent steps:
         // 64^3 tb2 / tb3 / tb4 29 / 24 / 33 -> 44 / 39 / 56 GLUPS)
@@ -123,11 +119,7 @@ static int tb2_launch(const SweepArgs& a) {
     int ych = a.tile.zch;
     if (ych <= 0) {
         ych = tb_chunk(a.ny, tiles_x, K, 0);
-        static long resident[16] = {};
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        long& res = resident[dev & 15];
-        if (!res) res = resident_groups((const void*)tb2d<T, P, S, V, K, W>, 64 * W);
+        const long res = cached_resident<tb2d<T, P, S, V, K, W>>(64 * W);
         if ((long)tiles_x * ((a.ny + ych - 1) / ych) < res)   // fewer workgroups than fit
             ych = fill_chunk(a.ny, tiles_x, res, ych, 2 * K, 2 * K);   // (1 row: 2048^2 tb4 -9%)
     }

@@ -74,11 +74,7 @@ static int zmarch_launch3(const SweepArgs& a) {
     const long tiles = (long)tiles_x * tiles_y;
     int zch = a.tile.zch;
     if (zch <= 0) {
-        static long resident[16] = {};
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        long& res = resident[dev & 15];
-        if (!res) res = resident_groups((const void*)zmarch3d<T, P, S, V, R, W>, 64 * W);
+        const long res = cached_resident<zmarch3d<T, P, S, V, R, W>>(64 * W);
         zch = fill_chunk(a.nz, tiles, res, 32, 2, 4);
     }
     const int chunks = (a.nz + zch - 1) / zch;
@@ -100,11 +96,7 @@ static int ymarch_launch2(const SweepArgs& a) {
     int ych = a.tile.zch;
     if (ych <= 0) {
         ych = pick_chunk(a.ny, tiles_x, 32, 2048);
-        static long resident[16] = {};
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        long& res = resident[dev & 15];
-        if (!res) res = resident_groups((const void*)ymarch2d<T, P, S, V, R, W>, 64 * W);
+        const long res = cached_resident<ymarch2d<T, P, S, V, R, W>>(64 * W);
         if ((long)tiles_x * ((a.ny + ych - 1) / ych) < res)   // fewer workgroups than fit
             ych = fill_chunk(a.ny, tiles_x, res, ych, 2, 4);
     }
