@@ -56,3 +56,14 @@ def test_registry_keys_and_json():
     json.dumps({k: v.description for k, v in VARIANTS.items()})
     assert VARIANTS["hip-cdna4/zmarch"].dims == (3,)
     assert VARIANTS["hip-cdna4/ymarch"].dims == (2,)
+
+
+def test_header_constants_match_the_binding():
+    """The Python mirrors of the header's #defines (kind codes, batch constants) agree."""
+    defines = {m.group(1): int(m.group(2)) for m in
+               re.finditer(r"#define (PH3_[A-Z_]+) \(?(-?\d+)\)?", HEADER.read_text())}
+    assert (defines["PH3_NAIVE"], defines["PH3_ZMARCH"], defines["PH3_TB"]) == (
+        _hip.NAIVE, _hip.ZMARCH, _hip.TB)
+    assert defines["PH3_BATCH_CHECK"] == _hip.BATCH_CHECK
+    assert defines["PH3_BATCH_SMALL_SITES"] == _hip.BATCH_SMALL_SITES
+    assert defines["PH3_STALL_RING"] == _hip.STALL_RING
