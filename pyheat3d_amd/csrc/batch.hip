@@ -7,9 +7,9 @@
 // workgroups through an atomic counter, so a single launch advances every small grid.
 // Tier 2, larger grids: one varlen launch per pass whose workgroups loop over the z-march (3D)
 // or y-march (2D) tiles of every running grid of one (storage, dims, bc, source) class, found by
-// binary search in a per-class tile prefix table. Every PH3_BATCH_CHECK passes three small
-// kernels advance the step counters, compute the residual of each Jacobi grid in fixed slices
-// and apply the convergence / stagnation / cap test.
+// binary search in a per-class tile prefix table. Every PH3_BATCH_CHECK passes two small
+// kernels compute the residual of each Jacobi grid in fixed slices, then advance the step
+// counters and apply the convergence / stagnation / cap test.
 // All per-site arithmetic is the canonical update of ph3_common.h, so a grid gets the same bits
 // whichever tier, tile or batch it runs in; tier and check points depend only on the grid.
 #include "march_tiles.h"
@@ -367,16 +367,11 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
     }
 }
 
-// After a segment of PH3_BATCH_CHECK passes: advance the step counters of running solves.
-__global__ void batch_advance(ph3_solve_desc* descs, const int* __restrict__ list, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    ph3_solve_desc& d = descs[list[t]];
-    if (d.status != 0) return;
+// Step count of a running solve once the current segment of PH3_BATCH_CHECK sweeps is done.
+__device__ inline long long done_after_segment(const ph3_solve_desc& d) {
     long long adv = d.steps - d.done;
     if (adv > PH3_BATCH_CHECK) adv = PH3_BATCH_CHECK;
-    d.done += adv;
-    if (d.mode == 0 && d.done >= d.steps) d.status = 1;
+    return d.done + adv;
 }
 
 template <typename T>
@@ -388,7 +383,9 @@ batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
         const int t = item / kSlices, sl = item - t * kSlices;
         const ph3_solve_desc& d = descs[list[t]];
         if (d.mode != 1 || d.status != 0 || d.storage != Codec<T>::code) continue;
-        const double part = residual_rows<T>((const T*)d.buf[cur_buf(d)], d.src, d.dims, d.nz,
+        // the state the segment's passes have written (the counters advance in batch_check)
+        const int cur = (int)((done_after_segment(d) / kpass_of(d)) & 1);
+        const double part = residual_rows<T>((const T*)d.buf[cur], d.src, d.dims, d.nz,
                                              d.ny, d.nx, d.periodic != 0, d.c0, d.c1,
                                              sl * (kWG / 64) + (threadIdx.x >> 6),
                                              kSlices * (kWG / 64), threadIdx.x & 63, 64);
@@ -397,12 +394,19 @@ batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
     }
 }
 
+// After a segment: advance the step counter of every running solve, finish heat solves that
+// reached their step count, and apply the convergence / stagnation / cap test to Jacobi ones.
 __global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list, int n,
                             const double* __restrict__ partial) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     ph3_solve_desc& d = descs[list[t]];
-    if (d.mode != 1 || d.status != 0) return;
+    if (d.status != 0) return;
+    d.done = done_after_segment(d);
+    if (d.mode != 1) {
+        if (d.done >= d.steps) d.status = 1;
+        return;
+    }
     double r = 0.0;
     for (int s = 0; s < kSlices; ++s) r += partial[t * kSlices + s];
     apply_check(d, r);
@@ -720,8 +724,6 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
                 PH3_CHECK(hipEventRecord(b->cls_join[i], b->cls[i]));
                 PH3_CHECK(hipStreamWaitEvent(s, b->cls_join[i], 0));
             }
-            ph3::batch_advance<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->n_large);
-            PH3_LAUNCH_CHECK("batch advance");
             bool st_used[3] = {false, false, false};
             for (const auto& c : b->classes) st_used[c.storage] = true;
             for (int st = 0; st < 3; ++st) {
