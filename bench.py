@@ -13,7 +13,8 @@ lattice updates per second over that region (GLUPS, inputs resident in HBM). The
 kernel's average launch time comes from HIP events recorded on the stepper's own stream around
 the same region. ``cpu_baseline`` times the NumPy oracle (one core) on a bounded sample.
 ``batch`` and ``serve`` carry the solves/s and p99 half of the metric on a bounded sample of the
-mixed batch workload (``--batch-n``; see ``batch_sample``). ``comparators`` times the reference's
+mixed batch workload (``--batch-n``; see ``batch_sample``). ``config_128_dirichlet`` runs the
+128^3 Dirichlet parity configuration in full on every float32 rung and on the NumPy oracle. ``comparators`` times the reference's
 own torch-eager GPU rung on the same device and field (``torch_eager_comparator``).
 """
 
@@ -314,6 +315,47 @@ def batch_sample(n: int) -> tuple[dict, dict]:
     return batch, serve
 
 
+def config1_leg(keys: list[str], steps: int = 100, repeats: int = 5) -> dict:
+    """The parity configuration at its full size: 128^3 float32, Dirichlet walls 0, interior
+    U[0,1) from default_rng(0), r = 0.15, 100 steps. The NumPy oracle runs it in full on one core
+    (the CPU baseline of this configuration and the bitwise check of every rung); each GPU rung
+    is timed as the median of ``repeats`` runs of ``steps`` steps (interior sites)."""
+    from oracle import numpy_ref as nr
+    from pyheat3d_amd.analytic import random_dirichlet
+    from pyheat3d_amd.problem import heat_stencil
+    from pyheat3d_amd.registry import make_stepper
+
+    u0 = random_dirichlet((128, 128, 128), 0)
+    st = heat_stencil(3, 0.15)
+    interior = 126 ** 3
+    t0 = time.perf_counter()
+    ref = nr.emulated_reference(u0, st.c0, st.c1, "dirichlet", steps)
+    cpu_s = time.perf_counter() - t0
+    out = {"workload": f"3D 7-point FTCS heat 128^3 fp32 Dirichlet (walls 0, interior U[0,1) "
+                       f"seed 0), r=0.15, {steps} steps",
+           "cpu_baseline": {"value": round(interior * steps / cpu_s / 1e9, 4), "unit": "GLUPS",
+                            "cores": 1, "kind": "port",
+                            "sample": f"NumPy oracle, the whole run ({cpu_s:.2f} s)"},
+           "rungs": {}}
+    for key in keys:
+        s = make_stepper(key, u0, st)
+        try:
+            s.step(steps)
+            s.synchronize()
+            bitwise = bool(np.array_equal(s.get(), ref))
+            times = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                s.step(steps)
+                s.synchronize()
+                times.append(time.perf_counter() - t0)
+        finally:
+            s.close()
+        out["rungs"][key] = {"glups": round(interior * steps / float(np.median(times)) / 1e9, 2),
+                             "bitwise_equal_to_oracle": bitwise}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -331,6 +373,8 @@ def main() -> int:
     ap.add_argument("--no-torch", action="store_true", help="skip the torch-eager comparator")
     ap.add_argument("--torch-child", nargs=3, type=int, metavar=("N", "STEPS", "WARMUP"),
                     help=argparse.SUPPRESS)
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the 128^3 Dirichlet parity configuration")
     ap.add_argument("--batch-n", type=int, default=512,
                     help="solves in the batched-vs-per-solve and served sample (0: skip)")
     a = ap.parse_args()
@@ -394,6 +438,14 @@ def main() -> int:
     pmc = load_traffic(f"zmarch_{a.n}")
     if pmc and pmc.get("variant") == a.variant and pmc.get("n") == a.n:
         traffic = round(pmc["hbm_bytes_per_launch"])
+    config1 = None
+    if not a.no_config1:
+        config1 = config1_leg(["hip-cdna4/naive", "hip-cdna4/zmarch", "hip-cdna4/tb2",
+                               "hip-cdna4/tb3", "hip-cdna4/tb4"])
+        log("config 128^3 Dirichlet 100 steps: " + ", ".join(
+            f"{k.split('/')[1]} {v['glups']:.0f}{'' if v['bitwise_equal_to_oracle'] else ' MISMATCH'}"
+            for k, v in config1["rungs"].items()) + f" GLUPS; NumPy oracle "
+            f"{config1['cpu_baseline']['value']:.3f} GLUPS on one core")
     batch = serve = None
     if a.batch_n > 0:
         batch, serve = batch_sample(a.batch_n)
@@ -419,6 +471,7 @@ def main() -> int:
                   "what": "amplitude of each sine mode after the run vs the closed form g_m^steps"},
         "host_transfer": head["host_transfer"],
         "rungs": others,
+        "config_128_dirichlet": config1,
         "comparators": {"torch-eager": torch_cmp},
         "batch": batch,
         "serve": serve,
