@@ -141,7 +141,11 @@ class HipStepper(Stepper):
             def record():
                 for _ in range(self.GRAPH_PASSES):
                     self._launch(self._kind_code, self.steps_per_pass)
-            g = self._graphs[key] = _hip.Graph(self.stream, record)
+            a, b = self.a, self.b
+            try:
+                g = self._graphs[key] = _hip.Graph(self.stream, record)
+            finally:   # recording swaps the buffers an even number of times, or fails midway
+                self.a, self.b = a, b
         return g
 
     def synchronize(self) -> None:
