@@ -56,6 +56,10 @@ def lookup(kind: str, k: int, dims: int, storage: str, bc: str,
     if not e:
         return None
     t = e["tile"]
+    # an entry serves every shape of its size bucket; one whose lane vector does not divide this
+    # row (tuned on 128^3, asked for 121^3) is not applied: the library's defaults pick a width
+    if t["vec"] and shape[-1] % t["vec"]:
+        return None
     return _hip.Tile(t["ty"], t["zch"], t["vec"], t["waves"], t.get("flags", 0))
 
 

@@ -146,6 +146,18 @@ def test_naive_every_lane_width(storage):
             assert np.array_equal(u, ref), (shape, vec, nr.rel_err(u, ref))
 
 
+def test_odd_widths_in_tuned_size_buckets():
+    """Grids whose size bucket has tuned tiles but whose odd row width the tuned lane vector
+    does not divide (121^3 beside the 128^3 entries) run on valid defaults, bit for bit."""
+    rng = np.random.default_rng(18)
+    for shape, bc in (((121, 121, 121), "dirichlet"), ((120, 124, 123), "periodic")):
+        u0 = rng.random(shape, dtype=np.float32)
+        st = heat_stencil(3, 0.15, bc)
+        ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 12)
+        for key in ("hip-cdna4/zmarch", "hip-cdna4/tb2", "hip-cdna4/tb3", "hip-cdna4/tb4"):
+            _check(key, _run(key, u0, st, 12), ref)
+
+
 @pytest.mark.parametrize("key,shape,bc", [
     ("hip-cdna4/zmarch", (40, 50, 64), "periodic"),
     ("hip-cdna4/tb3", (37, 41, 60), "dirichlet"),

@@ -175,6 +175,29 @@ def test_tuning_table_keys_resolve():
         "tb/k3/3d/float32/dirichlet/src/b9"
 
 
+def test_tuned_tile_needs_a_lane_vector_that_divides_the_row():
+    """An entry serves every shape of its size bucket; for a row its lane vector does not divide
+    (an odd width in the same bucket, e.g. 121^3 next to a 128^3 entry) it is not applied and the
+    library's defaults pick a valid width, instead of the sweep refusing the tile."""
+    from pyheat3d_amd import tune
+
+    checked = 0
+    for key, e in tune.load_table().items():
+        if e["tile"]["vec"] <= 1:
+            continue
+        kind, kk, dims, storage, bc = key.split("/")[:5]
+        shape = tuple(e["shape"])
+        odd = shape[:-1] + (shape[-1] - 1 if shape[-1] % 2 == 0 else shape[-1],)
+        if tune.bucket(odd) != tune.bucket(shape) or odd[-1] % 2 == 0:
+            continue
+        src = "/src/" in key
+        args = (kind, int(kk[1:]), int(dims[0]), storage, bc)
+        assert tune.lookup(*args, shape, src) is not None, key
+        assert tune.lookup(*args, odd, src) is None, key
+        checked += 1
+    assert checked > 10
+
+
 def test_bench_cpu_solve_seconds_runs_the_oracle():
     """The batch CPU baseline's per-solve timer (heat in full, Jacobi extrapolated)."""
     import bench
