@@ -381,7 +381,7 @@ def main() -> int:
     if a.torch_child:
         return torch_eager_child(*a.torch_child)
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
+        print(f"bench.py: this benchmark runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the multi-GPU z-slab design is documented, not built)", file=sys.stderr)
         return 2
 

@@ -4,7 +4,7 @@
  * src/pyheat3d/backends/stencil.cu with NVRTC and launches kernels with the argument tuple
  * (u, o, [nz,] ny, nx, c0, c1) (backends/cupy_backend.py:86-100, 183-186). Each entry point
  * below names the reference interface it replaces. Python binds them with ctypes
- * (pyheat3d_amd/_hip.py); INTEGRATION.md shows the binding a reference maintainer would add.
+ * (pyheat3d_amd/_hip.py).
  *
  * Conventions
  *   - Every function returns 0 on success or a negative PH3_E* code; ph3_last_error() gives a
@@ -31,7 +31,7 @@ extern "C" {
 #define PH3_EINVAL (-1)   /* bad argument (shape, storage, kind, alignment) */
 #define PH3_EHIP (-2)     /* a HIP runtime call failed */
 #define PH3_ENOMEM (-3)   /* device allocation failed */
-#define PH3_EUNSUP (-4)   /* valid request that this build does not support */
+#define PH3_EUNSUP (-4)   /* valid request that this library does not support */
 
 /* sweep kinds (the reference's rungs, backends/cupy_backend.py:115-194) */
 #define PH3_NAIVE 0   /* one site per thread            (stencil.cu:43-93 naive2d/naive3d) */

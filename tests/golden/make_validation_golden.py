@@ -1,7 +1,7 @@
 """Extract the FP64 validation numbers the reference committed (results/validation.json) into
 tests/golden/validation_golden.json, a data fixture for the oracle tests.
 
-Run in the build container only (the reference tree is not on the GPU box):
+Run where the reference tree is checked out (the GPU tests never read it):
     python tests/golden/make_validation_golden.py /root/reference/results/validation.json
 """
 

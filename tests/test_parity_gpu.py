@@ -2,7 +2,7 @@
 
 fp32 variants must equal the oracle bit for bit (FMA contraction is off in the library, and the
 per-site operation order is the oracle's); the test asserts bitwise equality and, as the
-north-star bound, a max relative deviation <= 1e-6. fp16/bf16 variants are compared with the
+stated fp32 bound, a max relative deviation <= 1e-6. fp16/bf16 variants are compared with the
 oracle that rounds to storage after every pass of k steps, within the reference's storage
 tolerances (validation.py:153-154).
 """
