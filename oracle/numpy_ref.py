@@ -16,6 +16,9 @@ How each function is pinned (checks live in ``tests/test_oracle.py``):
   sweep of float32 bit patterns.
 * :func:`emulated_reference` -- rounding cadence checked on a case where fp32 and storage
   results must differ, and k=1 equivalence with the per-step rounding.
+* :func:`window_reference` -- equals the same window of the full-grid oracle bit for bit in 2D
+  and 3D, both BCs, windows at corners, faces and across the periodic wrap; one layer of margin
+  too few is detected.
 * validation studies (:func:`fourier_run`, :func:`spatial_convergence`,
   :func:`temporal_convergence`, :func:`laplace_convergence`, :func:`laplace_quadratic`,
   :func:`poisson_convergence`) -- against the FP64 numbers the reference committed in
@@ -175,6 +178,35 @@ def emulated_reference(u0: np.ndarray, c0: float, c1: float, bc: str, steps: int
     for _ in range(rem):
         u = round_storage(reference_step(u, c0, c1, bc, bb), storage)
     return u
+
+
+def window_reference(u0: np.ndarray, lo: tuple[int, ...], size: int, c0: float, c1: float,
+                     bc: str, steps: int, margin: int | None = None
+                     ) -> tuple[np.ndarray, list[np.ndarray]]:
+    """fp32 oracle on one cubic window of a field after ``steps`` steps, without the full grid.
+
+    A site after t steps depends only on the sites within distance t of it (the light cone of
+    the 2d+1-point stencil), so the window padded by ``margin`` = t sites on each side (wrapped
+    around for periodic fields, clipped at the fixed Dirichlet walls) and stepped with its outer
+    layer held fixed (:func:`emulated_reference` with ``"dirichlet"``) is exact from t layers in.
+    ``lo`` is the window's first index per axis (periodic windows may run past the end and
+    wrap). Returns the window and, per axis, the indices of its sites in the full field.
+    """
+    n = u0.shape
+    m = steps if margin is None else margin
+    if bc == "periodic":
+        pad = [np.arange(a - m, a + size + m) % na for a, na in zip(lo, n)]
+        ref = emulated_reference(u0[np.ix_(*pad)], c0, c1, "dirichlet", steps)
+        core = tuple(slice(m, m + size) for _ in n)
+        return ref[core], [np.arange(a, a + size) % na for a, na in zip(lo, n)]
+    if any(a < 0 or a + size > na for a, na in zip(lo, n)):
+        raise ValueError("a Dirichlet window lies inside the grid")
+    start = [max(a - m, 0) for a in lo]
+    stop = [min(a + size + m, na) for a, na in zip(lo, n)]
+    ref = emulated_reference(u0[tuple(slice(a, b) for a, b in zip(start, stop))], c0, c1,
+                             "dirichlet", steps)
+    core = tuple(slice(a - s, a - s + size) for a, s in zip(lo, start))
+    return ref[core], [np.arange(a, a + size) for a in lo]
 
 
 def rel_err(u: np.ndarray, ref: np.ndarray) -> float:

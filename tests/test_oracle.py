@@ -213,6 +213,35 @@ def test_emulated_reference_cadence():
     assert np.array_equal(f32, cref.run_f32(u0, 0.2, 0.2, "periodic", 8))
 
 
+@pytest.mark.parametrize("bc", ["dirichlet", "periodic"])
+def test_window_reference_equals_the_full_grid_oracle(bc):
+    """Light-cone windows (corners, faces, centre, across the periodic wrap) equal the same
+    sites of the full-grid oracle bit for bit; with one layer of margin too few they do not."""
+    rng = np.random.default_rng(23)
+    steps, size = 5, 6
+    for shape in ((30, 31, 29), (33, 40)):
+        u0 = rng.random(shape, dtype=np.float32)
+        c0, c1 = 0.1, 0.9 / (2 * len(shape))
+        full = nr.emulated_reference(u0, c0, c1, bc, steps)
+        ends = [n - size for n in shape]
+        lows = [(0,) * len(shape), tuple(ends), tuple(n // 2 for n in ends),
+                tuple(0 if i % 2 else e for i, e in enumerate(ends))]
+        if bc == "periodic":
+            lows += [tuple(n - size // 2 for n in shape), tuple(n - 1 for n in shape)]
+        for lo in lows:
+            ref, idx = nr.window_reference(u0, lo, size, c0, c1, bc, steps)
+            assert ref.shape == (size,) * len(shape)
+            assert np.array_equal(ref, full[np.ix_(*idx)]), (shape, lo)
+        # a margin one layer short leaves stale values on the faces that are not walls
+        lo = tuple(n // 2 for n in ends)
+        short, idx = nr.window_reference(u0, lo, size, c0, c1, bc, steps, margin=steps - 1)
+        assert not np.array_equal(short, full[np.ix_(*idx)]), shape
+        if bc == "dirichlet":
+            with pytest.raises(ValueError):
+                nr.window_reference(u0, (shape[0] - 1,) + (0,) * (len(shape) - 1), size, c0, c1,
+                                    bc, steps)
+
+
 # -- FP64 residual ----------------------------------------------------------------------------
 
 @pytest.mark.parametrize("bc", ["dirichlet", "periodic"])

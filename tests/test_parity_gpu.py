@@ -302,6 +302,36 @@ def test_large_grid_every_3d_rung_matches_oracle(bc):
         assert bad == 0, (key, bc, bad, nr.rel_err(u, ref))
 
 
+@pytest.mark.parametrize("bc", ["periodic", "dirichlet"])
+def test_full_size_1024cube_rungs_agree_and_match_oracle_windows(bc):
+    """1024^3 float32 heat (r = 0.15), 10 steps, at the single-GPU benchmark size: every 3D rung
+    returns the same 4.3 GB field bit for bit (tb2 runs 5 fused passes, tb3 3 and one single
+    step, tb4 2 and two single steps), and windows of it at the corners, the centre and, for
+    periodic fields, across the wrap-around in every axis, equal the oracle bit for bit. The
+    full-field oracle runs too slowly on one core at this size; the windows use the light cone."""
+    n, steps, size = 1024, 10, 32
+    u0 = np.random.default_rng(1).random((n, n, n), dtype=np.float32)
+    st = heat_stencil(3, 0.15, bc)
+    lows = [(0, 0, 0), (n - size,) * 3, (n // 2 - size // 2,) * 3, (0, n // 2, n - size),
+            (n - size, 0, n // 3)]
+    if bc == "periodic":
+        lows += [(n - size // 2,) * 3, (n - 3, 5, n - size // 4)]
+    wins = [nr.window_reference(u0, lo, size, st.c0, st.c1, bc, steps) for lo in lows]
+    first = None
+    for key in ("hip-cdna4/naive", "hip-cdna4/zmarch", "hip-cdna4/tb2", "hip-cdna4/tb3",
+                "hip-cdna4/tb4"):
+        u = _run(key, u0, st, steps)
+        for lo, (ref, idx) in zip(lows, wins):
+            got = u[np.ix_(*idx)]
+            assert np.array_equal(got, ref), (key, bc, lo, nr.rel_err(got, ref))
+        if first is None:
+            first = u
+        else:
+            bad = int(np.count_nonzero(u != first))
+            assert bad == 0, (key, bc, bad, nr.rel_err(u, first))
+        del u
+
+
 def test_residual_matches_oracle():
     rng = np.random.default_rng(21)
     for shape in ((33, 40, 47), (61, 75)):
