@@ -181,30 +181,34 @@ def emulated_reference(u0: np.ndarray, c0: float, c1: float, bc: str, steps: int
 
 
 def window_reference(u0: np.ndarray, lo: tuple[int, ...], size: int, c0: float, c1: float,
-                     bc: str, steps: int, margin: int | None = None
+                     bc: str, steps: int, storage: str = "float32", k: int = 1,
+                     b: np.ndarray | None = None, margin: int | None = None
                      ) -> tuple[np.ndarray, list[np.ndarray]]:
-    """fp32 oracle on one cubic window of a field after ``steps`` steps, without the full grid.
+    """:func:`emulated_reference` on one cubic window of a field, without the full grid.
 
     A site after t steps depends only on the sites within distance t of it (the light cone of
-    the 2d+1-point stencil), so the window padded by ``margin`` = t sites on each side (wrapped
-    around for periodic fields, clipped at the fixed Dirichlet walls) and stepped with its outer
-    layer held fixed (:func:`emulated_reference` with ``"dirichlet"``) is exact from t layers in.
-    ``lo`` is the window's first index per axis (periodic windows may run past the end and
-    wrap). Returns the window and, per axis, the indices of its sites in the full field.
+    the 2d+1-point stencil; storage rounding and the source act site by site), so the window
+    padded by ``margin`` = t sites on each side (wrapped around for periodic fields, clipped at
+    the fixed Dirichlet walls) and stepped with its outer layer held fixed (``"dirichlet"``) is
+    exact from t layers in. ``lo`` is the window's first index per axis (periodic windows may
+    run past the end and wrap). Returns the window and, per axis, the indices of its sites in
+    the full field.
     """
     n = u0.shape
     m = steps if margin is None else margin
     if bc == "periodic":
-        pad = [np.arange(a - m, a + size + m) % na for a, na in zip(lo, n)]
-        ref = emulated_reference(u0[np.ix_(*pad)], c0, c1, "dirichlet", steps)
+        pad = np.ix_(*[np.arange(a - m, a + size + m) % na for a, na in zip(lo, n)])
+        ref = emulated_reference(u0[pad], c0, c1, "dirichlet", steps, storage, k,
+                                 None if b is None else b[pad])
         core = tuple(slice(m, m + size) for _ in n)
         return ref[core], [np.arange(a, a + size) % na for a, na in zip(lo, n)]
     if any(a < 0 or a + size > na for a, na in zip(lo, n)):
         raise ValueError("a Dirichlet window lies inside the grid")
     start = [max(a - m, 0) for a in lo]
     stop = [min(a + size + m, na) for a, na in zip(lo, n)]
-    ref = emulated_reference(u0[tuple(slice(a, b) for a, b in zip(start, stop))], c0, c1,
-                             "dirichlet", steps)
+    box = tuple(slice(a, e) for a, e in zip(start, stop))
+    ref = emulated_reference(u0[box], c0, c1, "dirichlet", steps, storage, k,
+                             None if b is None else b[box])
     core = tuple(slice(a - s, a - s + size) for a, s in zip(lo, start))
     return ref[core], [np.arange(a, a + size) for a in lo]
 

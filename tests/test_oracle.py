@@ -234,12 +234,28 @@ def test_window_reference_equals_the_full_grid_oracle(bc):
             assert np.array_equal(ref, full[np.ix_(*idx)]), (shape, lo)
         # a margin one layer short leaves stale values on the faces that are not walls
         lo = tuple(n // 2 for n in ends)
-        short, idx = nr.window_reference(u0, lo, size, c0, c1, bc, steps, margin=steps - 1)
+        short, idx = nr.window_reference(u0, lo, size, c0, c1, bc, steps,
+                                         margin=steps - 1)
         assert not np.array_equal(short, full[np.ix_(*idx)]), shape
         if bc == "dirichlet":
             with pytest.raises(ValueError):
                 nr.window_reference(u0, (shape[0] - 1,) + (0,) * (len(shape) - 1), size, c0, c1,
                                     bc, steps)
+
+
+def test_window_reference_with_storage_rounding_and_source():
+    """Windows of the bf16 / fp16 storage oracle with fused passes and a source term (the
+    Jacobi Poisson setting) equal the full-grid oracle's sites bit for bit."""
+    rng = np.random.default_rng(24)
+    shape, size = (26, 27, 28), 6
+    u0 = rng.random(shape, dtype=np.float32)
+    b = rng.random(shape, dtype=np.float32) * np.float32(1e-3)
+    for storage, k, steps in (("bfloat16", 4, 7), ("float16", 1, 6), ("float16", 4, 8)):
+        full = nr.emulated_reference(u0, 0.0, 1 / 6, "dirichlet", steps, storage, k, b)
+        for lo in ((0, 0, 0), (10, 11, 12), (20, 0, 22)):
+            ref, idx = nr.window_reference(u0, lo, size, 0.0, 1 / 6, "dirichlet", steps,
+                                           storage, k, b)
+            assert np.array_equal(ref, full[np.ix_(*idx)]), (storage, k, lo)
 
 
 # -- FP64 residual ----------------------------------------------------------------------------

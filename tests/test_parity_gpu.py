@@ -332,6 +332,38 @@ def test_full_size_1024cube_rungs_agree_and_match_oracle_windows(bc):
         del u
 
 
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+def test_full_size_768cube_jacobi_storage_matches_oracle_windows(storage):
+    """768^3 Jacobi sweeps with a float32 source and 16-bit storage (the Poisson solve's size and
+    settings), 10 sweeps from a random field: windows at the walls, the corners and the centre
+    equal the storage oracle (rounding after every pass of k sweeps) bit for bit, for the naive,
+    z-march and tb4 kernels; the two single-sweep kernels return the same field bit for bit."""
+    n, steps, size = 768, 10, 32
+    rng = np.random.default_rng(2)
+    u0 = rng.random((n, n, n), dtype=np.float32)
+    b = rng.random((n, n, n), dtype=np.float32) * np.float32(1e-3)
+    st = jacobi_stencil(3)
+    lows = [(0, 0, 0), (n - size,) * 3, (n // 2 - size // 2,) * 3, (0, n // 2, n - size)]
+    suffix = {"float16": "-fp16", "bfloat16": "-bf16"}[storage]
+    single = None
+    for key in ("hip-cdna4/naive", "hip-cdna4/zmarch", "hip-cdna4/tb4"):
+        key += suffix
+        k = VARIANTS[key].steps_per_pass
+        u = _run(key, u0, st, steps, b)
+        for lo in lows:
+            ref, idx = nr.window_reference(u0, lo, size, st.c0, st.c1, "dirichlet", steps,
+                                           storage, k, b)
+            got = u[np.ix_(*idx)]
+            assert np.array_equal(got, ref), (key, lo, nr.rel_err(got, ref))
+        if k == 1:
+            if single is None:
+                single = u
+            else:
+                bad = int(np.count_nonzero(u != single))
+                assert bad == 0, (key, bad)
+        del u
+
+
 def test_residual_matches_oracle():
     rng = np.random.default_rng(21)
     for shape in ((33, 40, 47), (61, 75)):
