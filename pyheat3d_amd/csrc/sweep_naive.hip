@@ -10,6 +10,33 @@ namespace ph3 {
 
 constexpr int kNaiveBX = 64, kNaiveBY = 4;
 
+// Column of site v of this thread and its x neighbours. Every index is clamped into the row
+// (columns past the end read the last site, Dirichlet walls read themselves), so all loads of a
+// thread are unconditional and the compiler issues the 7*VEC loads before the first wait; only
+// the store is guarded. A per-site guard around the loads would split the thread into VEC basic
+// blocks that each wait for their 7 loads before the next ones issue.
+template <bool P, int VEC>
+struct NaiveCols {
+    int i[VEC], ic[VEC], im[VEC], ip[VEC];
+    __device__ __forceinline__ NaiveCols(int nx) {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            i[v] = (blockIdx.x * VEC + v) * kNaiveBX + threadIdx.x;
+            ic[v] = min(i[v], nx - 1);
+            if (P) {
+                im[v] = ic[v] == 0 ? nx - 1 : ic[v] - 1;
+                ip[v] = ic[v] == nx - 1 ? 0 : ic[v] + 1;
+            } else {
+                im[v] = max(ic[v] - 1, 0);
+                ip[v] = min(ic[v] + 1, nx - 1);
+            }
+        }
+    }
+    __device__ __forceinline__ bool writes(int v, int nx) const {
+        return P ? i[v] < nx : (i[v] > 0 && i[v] < nx - 1);
+    }
+};
+
 template <typename T, bool P, bool S, int VEC>
 __global__ void __launch_bounds__(kNaiveBX * kNaiveBY)
 naive2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
@@ -24,21 +51,24 @@ naive2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
         jm = j - 1; jp = j + 1;
     }
     const size_t r = (size_t)j * nx, rm = (size_t)jm * nx, rp = (size_t)jp * nx;
+    const NaiveCols<P, VEC> x(nx);
+    float xm[VEC], xp[VEC], ym[VEC], yp[VEC], uc[VEC], bs[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
-        const int i = (blockIdx.x * VEC + v) * kNaiveBX + threadIdx.x;
-        if (i >= nx) continue;
-        int im, ip;
-        if (P) { im = i == 0 ? nx - 1 : i - 1; ip = i == nx - 1 ? 0 : i + 1; }
-        else {
-            if (i == 0 || i == nx - 1) continue;
-            im = i - 1; ip = i + 1;
-        }
-        float s = ld(u, r + im) + ld(u, r + ip);
-        s = s + (ld(u, rm + i) + ld(u, rp + i));
-        float w = update(c0, c1, ld(u, r + i), s);
-        if (S) w = w + b[r + i];
-        st(o, r + i, w);
+        xm[v] = ld(u, r + x.im[v]);
+        xp[v] = ld(u, r + x.ip[v]);
+        ym[v] = ld(u, rm + x.ic[v]);
+        yp[v] = ld(u, rp + x.ic[v]);
+        uc[v] = ld(u, r + x.ic[v]);
+        if (S) bs[v] = b[r + x.ic[v]];
+    }
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+        float s = xm[v] + xp[v];
+        s = s + (ym[v] + yp[v]);
+        float w = update(c0, c1, uc[v], s);
+        if (S) w = w + bs[v];
+        if (x.writes(v, nx)) st(o, r + x.i[v], w);
     }
 }
 
@@ -62,22 +92,27 @@ naive3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
     const size_t r = (size_t)k * plane + (size_t)j * nx;
     const size_t ry_m = (size_t)k * plane + (size_t)jm * nx, ry_p = (size_t)k * plane + (size_t)jp * nx;
     const size_t rz_m = (size_t)km * plane + (size_t)j * nx, rz_p = (size_t)kp * plane + (size_t)j * nx;
+    const NaiveCols<P, VEC> x(nx);
+    float xm[VEC], xp[VEC], ym[VEC], yp[VEC], zm[VEC], zp[VEC], uc[VEC], bs[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) {
-        const int i = (blockIdx.x * VEC + v) * kNaiveBX + threadIdx.x;
-        if (i >= nx) continue;
-        int im, ip;
-        if (P) { im = i == 0 ? nx - 1 : i - 1; ip = i == nx - 1 ? 0 : i + 1; }
-        else {
-            if (i == 0 || i == nx - 1) continue;
-            im = i - 1; ip = i + 1;
-        }
-        float s = ld(u, r + im) + ld(u, r + ip);
-        s = s + (ld(u, ry_m + i) + ld(u, ry_p + i));
-        s = s + (ld(u, rz_m + i) + ld(u, rz_p + i));
-        float w = update(c0, c1, ld(u, r + i), s);
-        if (S) w = w + b[r + i];
-        st(o, r + i, w);
+        xm[v] = ld(u, r + x.im[v]);
+        xp[v] = ld(u, r + x.ip[v]);
+        ym[v] = ld(u, ry_m + x.ic[v]);
+        yp[v] = ld(u, ry_p + x.ic[v]);
+        zm[v] = ld(u, rz_m + x.ic[v]);
+        zp[v] = ld(u, rz_p + x.ic[v]);
+        uc[v] = ld(u, r + x.ic[v]);
+        if (S) bs[v] = b[r + x.ic[v]];
+    }
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+        float s = xm[v] + xp[v];
+        s = s + (ym[v] + yp[v]);
+        s = s + (zm[v] + zp[v]);
+        float w = update(c0, c1, uc[v], s);
+        if (S) w = w + bs[v];
+        if (x.writes(v, nx)) st(o, r + x.i[v], w);
     }
 }
 
@@ -103,10 +138,12 @@ static int naive_launch(const SweepArgs& a) {
 
 int launch_naive(const SweepArgs& a) {
     // Sites per thread (the reference's NAIVE_VEC, backends/cupy_backend.py:53, which is 1 for
-    // 4-byte and 2 for 2-byte storage). Measured on MI355X (GLUPS, vec 1 / 2 / 4): float32 3D
-    // 1024^3 326 / 420 / 461, 2D 16384^2 397 / 565 / 536; bfloat16 3D 396 / 503 / 593, 2D
-    // 450 / 598 / 635. More independent loads in flight per lane is what pays.
-    int vec = a.tile.vec ? a.tile.vec : fit_vec(a.storage == kF32 && a.dims == 2 ? 2 : 4, a.nx);
+    // 4-byte and 2 for 2-byte storage). With every load of a thread issued before the first wait,
+    // more sites per lane is more loads in flight, and 4 wins for every storage type. Measured on
+    // MI355X (GLUPS, vec 1 / 2 / 4): float32 3D 1024^3 329 / 401 / 489, 2D 16384^2
+    // 403 / 550 / 711; bfloat16 3D 396 / 521 / 692, 2D 456 / 631 / 819. Rows narrower than 256
+    // sites take a narrower vector (fit_vec).
+    int vec = a.tile.vec ? a.tile.vec : fit_vec(4, a.nx);
     PH3_DISPATCH_TPS(a, {
         switch (vec) {
             case 1: return naive_launch<T, P, S, 1>(a);
