@@ -268,12 +268,14 @@ class BatchSolver:
                 if not running:
                     break
                 if self.compact and len(running) <= len(self.table) // 2:
-                    self.table = running
-                    self.batch.set([self.state[row] for row in running])
-                    compactions += 1
+                    # the round graph was recorded against the current table: destroy it
+                    # before the table changes under it
                     if graph is not None:
                         graph.close()
                         graph = None
+                    self.table = running
+                    self.batch.set([self.state[row] for row in running])
+                    compactions += 1
         finally:
             if graph is not None:
                 graph.close()

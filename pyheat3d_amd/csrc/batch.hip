@@ -421,23 +421,31 @@ struct Tier2Class {
     int vec = 1;   // sites per lane of the class's passes
     int few = 0;   // tier2_geom's short-chunk mode (set from the class's tile count)
     std::vector<int> idx;
-    int* d_list = nullptr;
-    long long* d_prefix = nullptr;
+    const int* d_list = nullptr;          // into the batch's int arena
+    const long long* d_prefix = nullptr;  // into the batch's prefix arena
 };
+
+// Upper bound on tier-2 classes: storage (3) x dims (2) x bc (2) x source (2) x kpass (4) x
+// lane width (2).
+constexpr int kMaxClasses = 192;
 
 struct ph3_batch {
     int cap = 0, n = 0;
     ph3_solve_desc* d_desc = nullptr;
     int* d_counter = nullptr;
     double* d_partial = nullptr;
-    int* d_small[3] = {nullptr, nullptr, nullptr};  // tier-1 lists per storage
+    // Per-set index tables live in two arenas sized for `cap` solves at creation: every list
+    // of a set (tier-1 lists, initial-residual lists, the tier-2 list, the per-class lists)
+    // together holds at most 4 * cap ints, and the per-class tile prefixes at most
+    // cap + kMaxClasses long longs. A set rewrites them in place and never frees device
+    // memory, so a graph recorded from an earlier round never points at freed memory.
+    int* d_iarena = nullptr;
+    long long* d_llarena = nullptr;
+    const int* d_small[3] = {nullptr, nullptr, nullptr};  // tier-1 lists per storage
     int n_small[3] = {0, 0, 0};
-    int* d_large = nullptr;                         // every tier-2 solve
+    const int* d_large = nullptr;                         // every tier-2 solve
     int n_large = 0;
-    int* d_init[3] = {nullptr, nullptr, nullptr};   // every solve per storage (initial residual)
-    int n_init[3] = {0, 0, 0};
     std::vector<Tier2Class> classes;
-    std::vector<void*> owned;                       // per-set device allocations
     hipStream_t side = nullptr;                     // tier 1 runs here, beside tier 2
     hipEvent_t fork = nullptr, join = nullptr;
     // tier-2 classes after the first run their passes on these, beside each other
@@ -445,37 +453,43 @@ struct ph3_batch {
     hipEvent_t seg = nullptr, cls_join[ph3::kClassStreams] = {};
 };
 
-static void free_set(ph3_batch* b) {
-    for (void* p : b->owned) (void)hipFree(p);
-    b->owned.clear();
+// Forget the current set (the batch then has no solves; rounds enqueue nothing).
+static void clear_set(ph3_batch* b) {
+    b->n = 0;
     b->classes.clear();
-    for (int s = 0; s < 3; ++s) { b->d_small[s] = b->d_init[s] = nullptr; b->n_small[s] = b->n_init[s] = 0; }
+    for (int s = 0; s < 3; ++s) { b->d_small[s] = nullptr; b->n_small[s] = 0; }
     b->d_large = nullptr;
     b->n_large = 0;
 }
 
-template <typename V>
-static int upload(ph3_batch* b, const std::vector<V>& v, V** out, hipStream_t s) {
-    *out = nullptr;
-    if (v.empty()) return PH3_OK;
-    void* p = nullptr;
-    PH3_CHECK(hipMalloc(&p, v.size() * sizeof(V)));
-    b->owned.push_back(p);
-    PH3_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, s));
-    *out = (V*)p;
-    return PH3_OK;
-}
-
 extern "C" size_t ph3_solve_desc_bytes(void) { return sizeof(ph3_solve_desc); }
 
+static void destroy_parts(ph3_batch* b) {
+    (void)hipFree(b->d_desc); (void)hipFree(b->d_counter); (void)hipFree(b->d_partial);
+    (void)hipFree(b->d_iarena); (void)hipFree(b->d_llarena);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->seg) (void)hipEventDestroy(b->seg);
+    for (int i = 0; i < ph3::kClassStreams; ++i) {
+        if (b->cls[i]) (void)hipStreamDestroy(b->cls[i]);
+        if (b->cls_join[i]) (void)hipEventDestroy(b->cls_join[i]);
+    }
+}
+
 extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
-    if (!out || capacity < 1) { set_error("ph3_batch_create: bad arguments"); return PH3_EINVAL; }
+    if (!out || capacity < 1 || capacity > (1 << 28)) {
+        set_error("ph3_batch_create: bad arguments (capacity %d)", capacity);
+        return PH3_EINVAL;
+    }
     ph3_batch* b = new ph3_batch();
     b->cap = capacity;
-    hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(ph3_solve_desc) * (size_t)capacity);
+    const size_t cap = (size_t)capacity;
+    hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(ph3_solve_desc) * cap);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, sizeof(int));
-    if (e == hipSuccess)
-        e = hipMalloc((void**)&b->d_partial, sizeof(double) * ph3::kSlices * (size_t)capacity);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_partial, sizeof(double) * ph3::kSlices * cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_iarena, sizeof(int) * 4 * cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_llarena, sizeof(long long) * (cap + kMaxClasses));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
@@ -485,15 +499,7 @@ extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
         if (e == hipSuccess) e = hipEventCreateWithFlags(&b->cls_join[i], hipEventDisableTiming);
     }
     if (e != hipSuccess) {
-        (void)hipFree(b->d_desc); (void)hipFree(b->d_counter); (void)hipFree(b->d_partial);
-        if (b->side) (void)hipStreamDestroy(b->side);
-        if (b->fork) (void)hipEventDestroy(b->fork);
-        if (b->join) (void)hipEventDestroy(b->join);
-        if (b->seg) (void)hipEventDestroy(b->seg);
-        for (int i = 0; i < ph3::kClassStreams; ++i) {
-            if (b->cls[i]) (void)hipStreamDestroy(b->cls[i]);
-            if (b->cls_join[i]) (void)hipEventDestroy(b->cls_join[i]);
-        }
+        destroy_parts(b);
         delete b;
         return ph3host::check(e, "ph3_batch_create");
     }
@@ -503,18 +509,8 @@ extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
 
 extern "C" int ph3_batch_destroy(ph3_batch* b) {
     if (!b) return PH3_OK;
-    free_set(b);
-    (void)hipFree(b->d_desc);
-    (void)hipFree(b->d_counter);
-    (void)hipFree(b->d_partial);
-    (void)hipStreamDestroy(b->side);
-    (void)hipEventDestroy(b->fork);
-    (void)hipEventDestroy(b->join);
-    (void)hipEventDestroy(b->seg);
-    for (int i = 0; i < ph3::kClassStreams; ++i) {
-        (void)hipStreamDestroy(b->cls[i]);
-        (void)hipEventDestroy(b->cls_join[i]);
-    }
+    clear_set(b);
+    destroy_parts(b);
     delete b;
     return PH3_OK;
 }
@@ -559,12 +555,13 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     for (int i = 0; i < n; ++i)
         if (int r = check_desc(descs[i], i)) return r;
     PH3_CHECK(hipStreamSynchronize(s));
-    free_set(b);
-    b->n = n;
+    // The new set is built in locals and committed only once every upload and the initial
+    // residuals have succeeded; from here until then the batch holds no solves.
+    clear_set(b);
     std::vector<ph3_solve_desc> table(descs, descs + n);
     for (auto& d : table) d.kpass = choose_kpass(d);
-    if (n) PH3_CHECK(hipMemcpyAsync(b->d_desc, table.data(), sizeof(ph3_solve_desc) * n, hipMemcpyHostToDevice, s));
     std::vector<int> small[3], init[3], large;
+    std::vector<Tier2Class> classes;
     for (int i = 0; i < n; ++i) {
         const ph3_solve_desc& d = table[i];
         init[d.storage].push_back(i);
@@ -578,51 +575,86 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
                         d.nx % 2 == 0 && al8(d.buf[0]) && al8(d.buf[1]) &&
                         (!d.src || al8(d.src)) ? 2 : 1;
         Tier2Class* c = nullptr;
-        for (auto& k : b->classes)
+        for (auto& k : classes)
             if (k.storage == d.storage && k.dims == d.dims && k.periodic == (d.periodic != 0) &&
                 k.src == (d.src != nullptr) && k.kpass == d.kpass && k.vec == vec) c = &k;
         if (!c) {
-            b->classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr,
-                                            d.kpass, vec, 0, {}});
-            c = &b->classes.back();
+            classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr,
+                                         d.kpass, vec, 0, {}});
+            c = &classes.back();
         }
         c->idx.push_back(i);
     }
-    for (int st = 0; st < 3; ++st) {
-        if (int r = upload(b, small[st], &b->d_small[st], s)) return r;
-        b->n_small[st] = (int)small[st].size();
-        if (int r = upload(b, init[st], &b->d_init[st], s)) return r;
-        b->n_init[st] = (int)init[st].size();
-    }
-    if (int r = upload(b, large, &b->d_large, s)) return r;
-    b->n_large = (int)large.size();
-    for (auto& c : b->classes) {
+    // host images of the two arenas; offsets are in elements
+    std::vector<int> ih;
+    std::vector<long long> lh;
+    auto put = [&ih](const std::vector<int>& v) { const size_t o = ih.size(); ih.insert(ih.end(), v.begin(), v.end()); return o; };
+    size_t o_small[3], o_init[3];
+    for (int st = 0; st < 3; ++st) { o_small[st] = put(small[st]); o_init[st] = put(init[st]); }
+    const size_t o_large = put(large);
+    std::vector<size_t> o_list(classes.size()), o_prefix(classes.size());
+    for (size_t ci = 0; ci < classes.size(); ++ci) {
+        Tier2Class& c = classes[ci];
         long long total = 0;
         for (int i : c.idx) {
             const ph3_solve_desc& d = table[i];
             total += ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, 0, c.vec).tiles;
         }
         c.few = total < ph3::kPassGroups ? 1 : 0;
-        std::vector<long long> prefix(c.idx.size() + 1, 0);
-        for (size_t k = 0; k < c.idx.size(); ++k) {
-            const ph3_solve_desc& d = table[c.idx[k]];
-            prefix[k + 1] = prefix[k] + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, c.vec).tiles;
+        o_list[ci] = put(c.idx);
+        o_prefix[ci] = lh.size();
+        lh.push_back(0);
+        for (int i : c.idx) {
+            const ph3_solve_desc& d = table[i];
+            lh.push_back(lh.back() + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, c.vec).tiles);
         }
-        if (int r = upload(b, c.idx, &c.d_list, s)) return r;
-        if (int r = upload(b, prefix, &c.d_prefix, s)) return r;
     }
+    if (ih.size() > 4 * (size_t)b->cap || lh.size() > (size_t)b->cap + kMaxClasses) {
+        set_error("ph3_batch_set: index tables exceed the batch's arenas");
+        return PH3_EINVAL;
+    }
+    // Uploads from these host vectors: the stream is synchronised before they go out of scope,
+    // on success and on every failure after the first enqueued copy.
+    auto fail = [s](hipError_t e, const char* what) {
+        (void)hipStreamSynchronize(s);
+        return ph3host::check(e, what);
+    };
+    hipError_t e = hipSuccess;
+    if (n) e = hipMemcpyAsync(b->d_desc, table.data(), sizeof(ph3_solve_desc) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !ih.empty())
+        e = hipMemcpyAsync(b->d_iarena, ih.data(), sizeof(int) * ih.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !lh.empty())
+        e = hipMemcpyAsync(b->d_llarena, lh.data(), sizeof(long long) * lh.size(), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return fail(e, "ph3_batch_set: upload");
     // initial residuals of unchecked Jacobi solves
     for (int st = 0; st < 3; ++st) {
-        if (!b->n_init[st]) continue;
-        const unsigned g = (unsigned)(b->n_init[st] < ph3::kPassGroups ? b->n_init[st] : ph3::kPassGroups);
+        const int ni = (int)init[st].size();
+        if (!ni) continue;
+        const int* list = b->d_iarena + o_init[st];
+        const unsigned g = (unsigned)(ni < ph3::kPassGroups ? ni : ph3::kPassGroups);
         switch (st) {
-            case ph3::kF32: ph3::batch_init_residual<float><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_init[st], b->n_init[st]); break;
-            case ph3::kF16: ph3::batch_init_residual<_Float16><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_init[st], b->n_init[st]); break;
-            default: ph3::batch_init_residual<ph3::bf16_t><<<g, ph3::kWG, 0, s>>>(b->d_desc, b->d_init[st], b->n_init[st]); break;
+            case ph3::kF32: ph3::batch_init_residual<float><<<g, ph3::kWG, 0, s>>>(b->d_desc, list, ni); break;
+            case ph3::kF16: ph3::batch_init_residual<_Float16><<<g, ph3::kWG, 0, s>>>(b->d_desc, list, ni); break;
+            default: ph3::batch_init_residual<ph3::bf16_t><<<g, ph3::kWG, 0, s>>>(b->d_desc, list, ni); break;
         }
-        PH3_LAUNCH_CHECK("batch initial residual");
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(e, "batch initial residual");
     }
-    PH3_CHECK(hipStreamSynchronize(s));
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ph3host::check(e, "ph3_batch_set: synchronize");
+    // commit
+    for (int st = 0; st < 3; ++st) {
+        b->n_small[st] = (int)small[st].size();
+        b->d_small[st] = b->n_small[st] ? b->d_iarena + o_small[st] : nullptr;
+    }
+    b->n_large = (int)large.size();
+    b->d_large = b->n_large ? b->d_iarena + o_large : nullptr;
+    for (size_t ci = 0; ci < classes.size(); ++ci) {
+        classes[ci].d_list = b->d_iarena + o_list[ci];
+        classes[ci].d_prefix = b->d_llarena + o_prefix[ci];
+    }
+    b->classes = std::move(classes);
+    b->n = n;
     return PH3_OK;
 }
 

@@ -106,10 +106,12 @@ class Server:
                 # 2. table upload and round graph
                 if dirty:
                     self.stream.sync()
-                    self.batch.set(descs)
+                    # the round graph was recorded against the previous table: destroy it
+                    # before the table changes under it
                     if graph is not None:
                         graph.close()
                         graph = None
+                    self.batch.set(descs)
                     if self.use_graph:
                         graph = _hip.Graph(self.stream, lambda: self.batch.round(self.round_sweeps))
                     dirty = False
