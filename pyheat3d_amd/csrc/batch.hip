@@ -45,6 +45,23 @@ __host__ __device__ inline int cur_buf(const ph3_solve_desc& d) {
     return (int)((d.done / kpass_of(d)) & 1);
 }
 
+// The problem fields of a descriptor that the workgroup-level kernels read, as scalars. (A copy
+// of the whole descriptor indexed by a run-time buffer number lives in scratch memory: 400 bytes
+// per lane.)
+struct Prob {
+    int dims, nz, ny, nx, mode;
+    bool P;
+    float c0, c1;
+    const float* src;
+    void* b0;
+    void* b1;
+    long long steps;
+    __device__ explicit Prob(const ph3_solve_desc& d)
+        : dims(d.dims), nz(d.nz), ny(d.ny), nx(d.nx), mode(d.mode), P(d.periodic != 0),
+          c0(d.c0), c1(d.c1), src(d.src), b0(d.buf[0]), b1(d.buf[1]), steps(d.steps) {}
+    __device__ void* buf(long long i) const { return (i & 1) ? b1 : b0; }
+};
+
 // ---- one workgroup: sweep and residual of a whole grid ---------------------------------------
 
 // One sweep of a whole grid (at most PH3_BATCH_SMALL_SITES sites, so offsets fit in 32 bits)
@@ -159,9 +176,9 @@ __device__ double wg_sum(double v, double* sm) {
 }
 
 template <typename T>
-__device__ double wg_residual(const ph3_solve_desc& d, const void* u, double* sm) {
-    const double part = residual_rows<T>((const T*)u, d.src, d.dims, d.nz, d.ny, d.nx,
-                                         d.periodic != 0, d.c0, d.c1, threadIdx.x >> 6, kWG / 64,
+__device__ double wg_residual(const Prob& d, const void* u, double* sm) {
+    const double part = residual_rows<T>((const T*)u, d.src, d.dims, d.nz, d.ny, d.nx, d.P,
+                                         d.c0, d.c1, threadIdx.x >> 6, kWG / 64,
                                          threadIdx.x & 63, 64);
     return wg_sum(part, sm);
 }
@@ -197,7 +214,7 @@ __device__ void small_solve(ph3_solve_desc* dp, int sweeps, double* sm) {
     __syncthreads();
     long long done = s_done;
     int status = s_status;
-    const ph3_solve_desc d = *dp;  // read-only copy of the problem fields
+    const Prob d(*dp);  // the problem fields
     if (status != 0) return;
     const long long target = done + sweeps;
     while (status == 0 && done < target) {
@@ -210,15 +227,15 @@ __device__ void small_solve(ph3_solve_desc* dp, int sweeps, double* sm) {
         }
         for (long long i = 0; i < seg; ++i) {
             const long long cur = (done + i) & 1;
-            wg_sweep<T>((const T*)d.buf[cur], (T*)d.buf[cur ^ 1], d.src, d.dims, d.nz, d.ny, d.nx,
-                        d.periodic != 0, d.c0, d.c1);
+            wg_sweep<T>((const T*)d.buf(cur), (T*)d.buf(cur ^ 1), d.src, d.dims, d.nz, d.ny, d.nx,
+                        d.P, d.c0, d.c1);
             __syncthreads();
         }
         done += seg;
         if (d.mode == 0) {
             if (done >= d.steps) status = 1;
         } else {
-            const double r = wg_residual<T>(d, d.buf[done & 1], sm);
+            const double r = wg_residual<T>(d, d.buf(done), sm);
             if (threadIdx.x == 0) {
                 dp->done = done;
                 apply_check(*dp, r);
@@ -261,9 +278,9 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
     __shared__ double sm[kWG];
     for (int t = blockIdx.x; t < n; t += gridDim.x) {
         ph3_solve_desc* dp = descs + list[t];
-        const ph3_solve_desc d = *dp;
-        if (d.mode != 1 || d.checks != 0) continue;   // uniform over the workgroup
-        const double r = wg_residual<T>(d, d.buf[cur_buf(d)], sm);
+        if (dp->mode != 1 || dp->checks != 0) continue;   // uniform over the workgroup
+        const Prob d(*dp);
+        const double r = wg_residual<T>(d, d.buf(dp->done / kpass_of(*dp)), sm);
         if (threadIdx.x == 0) {
             dp->res0_sq = r;
             dp->res_sq = r;

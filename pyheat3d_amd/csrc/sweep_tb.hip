@@ -30,8 +30,9 @@ tb3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
 {
     const int L = tb_logical_tile(remap);
     const int rest = L / tiles_x;
-    tb3d_tile<T, P, S, V, R, W, K>(u, o, b, nz, ny, nx, c0, c1, zch, L % tiles_x,
-                                   rest % tiles_y, rest / tiles_y);
+    // store modes measured per k at 1024^3 (see StoreMode)
+    tb3d_tile<T, P, S, V, R, W, K, K == 4 ? kStorePlanes : kStoreMasked>(
+        u, o, b, nz, ny, nx, c0, c1, zch, L % tiles_x, rest % tiles_y, rest / tiles_y);
 }
 
 template <typename T, bool P, bool S, int V, int K, int W>
@@ -40,7 +41,8 @@ tb2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
      int ny, int nx, float c0, float c1, int ych, int tiles_x, int remap)
 {
     const int L = tb_logical_tile(remap);
-    tb2d_tile<T, P, S, V, K, W>(u, o, b, ny, nx, c0, c1, ych, L % tiles_x, L / tiles_x);
+    tb2d_tile<T, P, S, V, K, W, kStoreAll>(u, o, b, ny, nx, c0, c1, ych, L % tiles_x,
+                                           L / tiles_x);
 }
 
 // ---- launch ---------------------------------------------------------------------------------
@@ -194,6 +196,14 @@ static int tb_k(const SweepArgs& a) {
 
 int launch_tb(const SweepArgs& a, int k) {
     if (a.tile.flags) { ph3host::set_error("tb: tile flags must be 0"); return PH3_EINVAL; }
+    // the tiles store through buffer resources over one plane (3D) or row (2D)
+    const size_t elem = a.storage == kF32 ? 4 : 2;
+    const size_t span = (a.dims == 3 ? (size_t)a.ny : 1) * (size_t)a.nx * elem;
+    if (span > kMaxBufferBytes) {
+        ph3host::set_error("tb: a %s of %zu bytes exceeds the %zu-byte store window",
+                           a.dims == 3 ? "plane" : "row", span, (size_t)kMaxBufferBytes);
+        return PH3_EUNSUP;
+    }
     switch (k) {
         case 2: return tb_k<2>(a);
         case 3: return tb_k<3>(a);

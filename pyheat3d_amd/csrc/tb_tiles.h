@@ -32,10 +32,27 @@ template <int V, int K> struct TbGeom {
     static constexpr int OX = 64 * V - 2 * KV;  // output sites per tile along x
 };
 
+// How a tile stores its output rows (measured on MI355X; the choice never changes the bits):
+//  kStoreMasked  global stores behind exec-masked branches (lanes and planes that do not write
+//                skip them). The compiler cannot count stores behind a branch, so the wait for
+//                the next plane's prefetch also waits for them. Best for the batched varlen
+//                passes, whose short tiles pay for the entry wait of the other modes (bench
+//                batch mix: 970 vs 815-945 solves/s).
+//  kStoreAll     a raw buffer store of every row on every step; lanes and planes that must not
+//                write get the drop offset or a 0-byte buffer. The waits count the stores
+//                exactly. 2D single grids at 16384^2: tb8 4047 -> 4245 GLUPS (vec 2), 2771 ->
+//                3187 (vec 1); 3D tb2 / tb3 no better.
+//  kStorePlanes  buffer stores on the planes that are stored (a uniform branch), drop offsets
+//                for lanes. 3D tb4 at 1024^3: 1747 -> 1812 GLUPS periodic, 1537 -> 1597
+//                Dirichlet.
+// The buffer-store modes make the wave wait for the tile's first prefetch before its loop (see
+// arrived()), so that the loop entry and the back edge look alike to the wait counting.
+enum StoreMode : int { kStoreMasked = 0, kStoreAll = 1, kStorePlanes = 2 };
+
 // One overlapped tile (tx, ty) of z chunk cz, shared with the batched varlen kernels. Every
 // wave reaches every barrier (no early return). A caller running several tiles in a row must
 // put a barrier between them (the LDS slot is reused).
-template <typename T, bool P, bool S, int V, int R, int W, int K>
+template <typename T, bool P, bool S, int V, int R, int W, int K, int SM = kStoreMasked>
 __device__ __forceinline__ void
 tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
           int nz, int ny, int nx, float c0, float c1, int zch, int tx, int ty, int cz)
@@ -116,6 +133,14 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
             else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
             if constexpr (S) load_vec<float, V>(b, zb + roff[r] + lx, bnext[r]);
+        }
+        if constexpr (SM != kStoreMasked) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (kDirect) arrived(pref[r]);
+                else arrived(prefp[r]);
+                if constexpr (S) arrived(bnext[r]);
+            }
         }
     }
     int par = 0;   // LDS slot parity, flipped once per stage
@@ -215,13 +240,26 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         }
         const int c = iz - K;
         const bool zok = c >= z0 && c < z1 && (P || (c > 0 && c < nz - 1));
-        if (zok && wmask) {
-            const size_t zo = (size_t)c * plane;
+        constexpr uint32_t full = (1u << V) - 1u;
+        if constexpr (SM == kStoreMasked) {
+            if (zok && wmask) {
+                const size_t zo = (size_t)c * plane;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (((wmask >> (r * V)) & full) == full)
+                        store_vec<T, V>(o, zo + roff[r] + gx, res[r]);
+            }
+        } else if (SM == kStoreAll || zok) {
+            // kStoreAll: every step issues the same R stores (a plane outside the chunk, or a
+            // fixed one, gets a 0-byte buffer); lanes outside the output region get the drop
+            // offset
+            const __amdgpu_buffer_rsrc_t ob = buffer_over(
+                o + (zok ? (size_t)c * plane : 0), zok ? (uint32_t)(plane * sizeof(T)) : 0u);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                constexpr uint32_t full = (1u << V) - 1u;
-                if (((wmask >> (r * V)) & full) == full)
-                    store_vec<T, V>(o, zo + roff[r] + gx, res[r]);
+                const bool w = ((wmask >> (r * V)) & full) == full;
+                store_vec_buf<T, V>(ob, w ? (uint32_t)((roff[r] + gx) * sizeof(T)) : kDropOffset,
+                                    res[r]);
             }
         }
     }
@@ -229,7 +267,8 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 }
 
 // Column group gx (W strips) of row chunk cy; waves are independent (no LDS, no barrier).
-template <typename T, bool P, bool S, int V, int K, int W>
+// SM: the StoreMode of tb3d_tile (kStorePlanes skips the stores of rows that are not stored).
+template <typename T, bool P, bool S, int V, int K, int W, int SM = kStoreMasked>
 __device__ __forceinline__ void
 tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
           int ny, int nx, float c0, float c1, int ych, int gxg, int cy)
@@ -275,6 +314,11 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     }
     if constexpr (kDirect) load_vec<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx, pref);
     else prefp = load_packed<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx);
+    if constexpr (SM != kStoreMasked) {   // loop entry like the back edge (see arrived())
+        if constexpr (kDirect) arrived(pref);
+        else arrived(prefp);
+        if constexpr (S) arrived(bnext);
+    }
     for (int ib = y0 - K; ib < y1 + K; ib += 3) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -328,9 +372,16 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                     if (yfixed || ((fixmask >> e) & 1u)) dst[e] = ctr[e];
             }
         }
+        // wmask is all V bits or none
         const int c = iy - K;
-        if (wmask && c >= y0 && c < y1 && (P || (c > 0 && c < ny - 1))) {
-            store_vec<T, V>(o, (size_t)c * nx + gx, res);   // wmask is all V bits or none
+        const bool yok = c >= y0 && c < y1 && (P || (c > 0 && c < ny - 1));
+        if constexpr (SM == kStoreMasked) {
+            if (wmask && yok) store_vec<T, V>(o, (size_t)c * nx + gx, res);
+        } else if (SM == kStoreAll || yok) {
+            // kStoreAll: a store on every step, dropped outside the output rows and lanes
+            const __amdgpu_buffer_rsrc_t ob =
+                buffer_over(o + (yok ? (size_t)c * nx : 0), yok ? (uint32_t)(nx * sizeof(T)) : 0u);
+            store_vec_buf<T, V>(ob, wmask ? (uint32_t)(gx * sizeof(T)) : kDropOffset, res);
         }
     }
     }

@@ -42,6 +42,67 @@ __device__ __forceinline__ void store_vec(T* __restrict__ p, size_t i, const flo
     }
 }
 
+// Masked stores without a branch. A raw buffer store whose offset lies at or past the buffer's
+// num_records bytes is dropped by the hardware, so lanes (or whole steps) that must not write
+// get kDropOffset instead of being masked off. Every step of a loop then issues the same number
+// of stores, and the compiler's waits for loads issued before those stores stay exact (behind
+// an exec-masked branch it cannot count them and waits for the stores as well).
+constexpr uint32_t kDropOffset = 0x80000000u;
+constexpr size_t kMaxBufferBytes = 0x7FFFFFFFu;   // num_records must stay below kDropOffset
+
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// Buffer resource over `bytes` bytes from p (0 bytes: every store through it is dropped).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void* p, uint32_t bytes) {
+    // word 3: 32-bit data format, the gfx9-family value for raw (untyped) access
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_vec_buf(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes,
+                                              const float (&v)[V]) {
+    constexpr int B = V * (int)sizeof(T);
+    static_assert(B == 2 || B == 4 || B == 8 || B == 16, "buffer stores of 2-16 bytes");
+    T tmp[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) tmp[e] = Codec<T>::store(v[e]);
+    if constexpr (B == 2) {
+        unsigned short x;
+        __builtin_memcpy(&x, tmp, 2);
+        __builtin_amdgcn_raw_buffer_store_b16(x, r, off_bytes, 0, 0);
+    } else if constexpr (B == 4) {
+        unsigned int x;
+        __builtin_memcpy(&x, tmp, 4);
+        __builtin_amdgcn_raw_buffer_store_b32(x, r, off_bytes, 0, 0);
+    } else if constexpr (B == 8) {
+        u32x2_t x;
+        __builtin_memcpy(&x, tmp, 8);
+        __builtin_amdgcn_raw_buffer_store_b64(x, r, off_bytes, 0, 0);
+    } else {
+        u32x4_t x;
+        __builtin_memcpy(&x, tmp, 16);
+        __builtin_amdgcn_raw_buffer_store_b128(x, r, off_bytes, 0, 0);
+    }
+}
+
+// Make the wave wait here until a loaded value has arrived: an empty asm statement that reads
+// the register counts as a use for the compiler's wait insertion. Applied to a loop's first
+// prefetch, it makes the loop entry look like the back edge (loads complete, stores possibly in
+// flight); otherwise the merge of the two at the loop header takes the entry's pending loads as
+// the worst case and the header waits for every store the previous iteration left in flight.
+__device__ __forceinline__ void arrived(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void arrived(uint32_t v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void arrived(uint16_t v) { arrived((uint32_t)v); }
+__device__ __forceinline__ void arrived(const uint2& v) { arrived(v.x); arrived(v.y); }
+__device__ __forceinline__ void arrived(const uint4& v) {
+    arrived(v.x); arrived(v.y); arrived(v.z); arrived(v.w);
+}
+template <int V> __device__ __forceinline__ void arrived(const float (&v)[V]) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) arrived(v[e]);
+}
+
 // A lane vector kept in its stored format between the load and the step that uses it: a
 // conversion placed right after a load would make the wave wait for that load there.
 template <typename T, int V> using Packed = typename RawVec<V * sizeof(T)>::type;
