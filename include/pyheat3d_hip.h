@@ -126,6 +126,19 @@ int ph3_residual(int storage, int periodic, int dims, int nz, int ny, int nx, fl
                  float c1, const void* u, const float* b, double* out, void* work,
                  void* stream);
 
+/* ---- the FP64 residual without any allocation. No counterpart in the reference: this entry
+ *      point is for callers that cannot allocate device memory (code under HIP graph capture,
+ *      tight serving loops, harnesses that forbid hipMalloc). Same arguments, kernels and result
+ *      bits as ph3_residual, minus `work`: the pass-1 partials live in a static device array of
+ *      ph3_residual_noalloc_scratch_bytes() bytes (8 KiB, one FP64 partial per pass-1
+ *      workgroup; every grid shape fits, since that count is fixed) that each device holds
+ *      until the process ends. The first call on a device looks up that device's copy. Calls
+ *      never allocate; calls that overlap on one device race on the shared partials, so order
+ *      them on one stream (or synchronise between streams). ------------------------------ */
+size_t ph3_residual_noalloc_scratch_bytes(void);
+int ph3_residual_noalloc(int storage, int periodic, int dims, int nz, int ny, int nx, float c0,
+                         float c1, const void* u, const float* b, double* out, void* stream);
+
 /* ---- HIP graphs: capture whatever the calling thread enqueues on `stream` between begin and
  *      end into an executable graph (no counterpart in the reference; its nearest analogue is
  *      Warp's pre-recorded launches, backends/warp_backend.py:144-153). ---------------------- */

@@ -69,6 +69,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ph3_residual_work_bytes": (c_size_t, []),
     "ph3_residual": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_float,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph3_residual_noalloc_scratch_bytes": (c_size_t, []),
+    "ph3_residual_noalloc": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_float,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph3_graph_begin": (c_int, [c_void_p]),
     "ph3_graph_end": (c_int, [c_void_p, POINTER(c_void_p)]),
     "ph3_graph_launch": (c_int, [c_void_p, c_void_p]),

@@ -7,6 +7,8 @@
 // after the same number of sweeps alone or inside a batch.
 #include "sweep_impl.h"
 
+#include <mutex>
+
 namespace ph3 {
 
 constexpr int kResBlocks = 1024;
@@ -96,4 +98,53 @@ extern "C" int ph3_residual(int storage, int periodic, int dims, int nz, int ny,
     ph3::residual_pass2<<<1, ph3::kResThreads, 0, a.stream>>>(partial, out);
     PH3_LAUNCH_CHECK("residual pass 2");
     return PH3_OK;
+}
+
+// ---- the residual without any allocation ---------------------------------------------------
+// Pass 1's workgroup partials live in this file-scope device array instead of a caller
+// workspace. Every device holds its own copy at its own address, kept per device after the
+// first call on it; host code passes that address to the kernels, never the array's name
+// (which is a host-side stand-in). Pass 1 overwrites every partial, so nothing needs clearing.
+static __device__ __align__(256) double scratch[ph3::kResBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_of_current_device(double** out) {
+    int dev = 0;
+    PH3_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        ph3host::set_error("residual_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return PH3_EUNSUP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return ph3host::check(e, "hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = (double*)p;
+    }
+    *out = g_scratch[dev];
+    return PH3_OK;
+}
+}  // namespace
+
+extern "C" size_t ph3_residual_noalloc_scratch_bytes(void) { return sizeof(scratch); }
+
+extern "C" int ph3_residual_noalloc(int storage, int periodic, int dims, int nz, int ny, int nx,
+                                    float c0, float c1, const void* u, const float* b,
+                                    double* out, void* stream)
+{
+    // The kernels need one partial per pass-1 workgroup, a fixed count whatever the shape.
+    static_assert(sizeof(scratch) == sizeof(double) * ph3::kResBlocks, "one partial per workgroup");
+    if (ph3_residual_work_bytes() > sizeof(scratch)) {
+        ph3host::set_error("residual_noalloc: %zu bytes of partials exceed the %zu-byte scratch",
+                           ph3_residual_work_bytes(), sizeof(scratch));
+        return PH3_EUNSUP;
+    }
+    double* partial = nullptr;
+    if (int r = scratch_of_current_device(&partial)) return r;
+    return ph3_residual(storage, periodic, dims, nz, ny, nx, c0, c1, u, b, out, partial, stream);
 }
