@@ -224,20 +224,25 @@ def torch_eager_comparator(u0: np.ndarray, stencil, steps: int, warmup: int) -> 
 def _cpu_solve_seconds(args: tuple) -> float:
     """One-core NumPy-oracle time of one batch request (module level, for a process pool).
 
-    Heat requests run all their steps. Jacobi requests time 20 sweeps and one FP64 residual and
-    extrapolate to the sweep count the GPU run reached, with a residual every 64 sweeps.
+    A solve is timed in full when it is at most 5e7 site updates; otherwise 5 steps (heat) or
+    20 sweeps plus one FP64 residual (Jacobi) are timed and scaled to the step count of the
+    GPU run (Jacobi: with a residual every 64 sweeps).
     """
     from oracle import numpy_ref as nr
     from pyheat3d_amd.analytic import splitmix_field
 
     shape, seed, kind, c0, c1, mode, sweeps = args
     s = nr.NumpyStepper(splitmix_field(shape, seed, kind), c0, c1, "dirichlet")
-    t0 = time.perf_counter()
+    sites = math.prod(shape)
     if mode == "heat":
-        s.step(sweeps)
-        return time.perf_counter() - t0
-    s.step(20)
-    t_sweep = (time.perf_counter() - t0) / 20
+        part = sweeps if sites * sweeps <= 5e7 else 5
+        t0 = time.perf_counter()
+        s.step(part)
+        return (time.perf_counter() - t0) * sweeps / part
+    part = min(20, sweeps) if sites * sweeps > 5e7 else sweeps
+    t0 = time.perf_counter()
+    s.step(part)
+    t_sweep = (time.perf_counter() - t0) / max(part, 1)
     t0 = time.perf_counter()
     nr.residual_fp64(s.get(), c1, "dirichlet")
     t_res = time.perf_counter() - t0
@@ -264,54 +269,76 @@ def cpu_batch_baseline(reqs: list, sweeps: dict[int, int]) -> dict:
             "solves_per_s_1core": round(len(jobs) / sum(one), 4),
             "solves_per_s_all_cores": round(len(jobs) * cores / sum(many), 4),
             "cores": cores,
-            "sample": "NumPy oracle (Dirichlet slicing stepper) on the per-solve sample; heat "
-                      "solves run in full, Jacobi solves timed over 20 sweeps + one FP64 residual "
-                      "and extrapolated to the GPU run's sweep count; all-core figure assumes "
-                      f"independent solves on {cores} processes ({measured:.1f} s measured)"}
+            "sample": "NumPy oracle (Dirichlet slicing stepper) on the per-solve sample, the "
+                      "GPU run's step counts; solves over 5e7 site updates timed on 5 steps "
+                      "(heat) or 20 sweeps + one FP64 residual (Jacobi) and scaled; all-core "
+                      f"figure assumes independent solves on {cores} processes ({measured:.1f} s "
+                      "measured)"}
+
+
+#: the bench's sample of the mixed batch workload: config 4's size ranges, a sweep cap that
+#: keeps the default run within minutes
+BATCH_N2, BATCH_N3, BATCH_CAP = (64, 2048), (32, 256), 16384
 
 
 def batch_sample(n: int) -> tuple[dict, dict]:
     """The solves/s + p99 half of the metric on a bounded sample of the batch mix.
 
-    ``n`` mixed solves (60% 2D N in [64, 256], 40% 3D N in [32, 64], half 200 heat steps, half
-    Jacobi Laplace to a 1e-6 relative FP64 residual, float32): all submitted at once, batched
-    graph rounds vs one launch per sweep per solve (every 16th solve; also with the stepper's
-    replayed graphs of 16 sweeps), then the same mix served
-    at 1000 Poisson arrivals per second for latency percentiles. The full-size mixes are in
+    ``n`` mixed solves drawn from the mixed workload's full size ranges (60% 2D N in
+    [64, 2048], 40% 3D N in [32, 256], half 200 heat steps, half Jacobi Laplace to a 1e-6
+    relative FP64 residual with a sweep cap of ``BATCH_CAP``, float32): all submitted at once,
+    batched graph rounds vs one launch per pass per solve on the batch's own fused-step rung
+    (every 16th solve; also with the stepper's replayed graphs), then the same mix served at
+    1000 Poisson arrivals per second. Solves that reached their target are counted apart from
+    those that stagnated at the float32 floor or hit the cap; every 32nd result of bounded size
+    is compared with its FP64 NumPy solution. The full mixes run under
     ``python -m pyheat3d_amd batch|serve``.
     """
+    from pyheat3d_amd.batch import make_mix
     from pyheat3d_amd.benchmark import batch_benchmark, serve_benchmark
 
-    n2, n3 = (64, 256), (32, 64)
     t0 = time.perf_counter()
-    b = batch_benchmark(n, seed=2024, n2=n2, n3=n3, frac_jacobi=0.5, baseline_every=16)
-    log(f"batch: {b['batched']['solves_per_s']:.1f} solves/s batched vs "
-        f"{b['per_solve']['solves_per_s']:.2f} per-solve ({b['speedup']:.1f}x; "
+    b = batch_benchmark(n, seed=2024, n2=BATCH_N2, n3=BATCH_N3, frac_jacobi=0.5,
+                        baseline_every=16, jacobi_cap=BATCH_CAP, accuracy_every=32)
+    bb, ps = b["batched"], b["per_solve"]
+    log(f"batch: {bb['solves_per_s']:.1f} solves/s batched ({bb['reached_target']['solves']} "
+        f"reached the target) vs {ps['solves_per_s']:.2f} per-solve ({b['speedup']:.1f}x; "
         f"{b['per_solve_graphs']['solves_per_s']:.2f} with replayed graphs, "
         f"{b['speedup_vs_graphs']:.1f}x) in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    from pyheat3d_amd.batch import make_mix
-    sample = make_mix(n, 2024, n2=n2, n3=n3, frac_jacobi=0.5)[::16]
+    sample = make_mix(n, 2024, n2=BATCH_N2, n3=BATCH_N3, frac_jacobi=0.5,
+                      jacobi_cap=BATCH_CAP)[::16]
     cpu = cpu_batch_baseline(sample, b["sample_sweeps"])
     log(f"batch cpu baseline: {cpu['solves_per_s_1core']:.3f} solves/s on 1 core, "
         f"{cpu['solves_per_s_all_cores']:.3f} on {cpu['cores']} in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    s = serve_benchmark(n, rate=1000.0, seed=2024, n2=n2, n3=n3)
+    s = serve_benchmark(n, rate=1000.0, seed=2024, n2=BATCH_N2, n3=BATCH_N3,
+                        jacobi_cap=BATCH_CAP, accuracy_every=32)
     log(f"serve: {s['solves_per_s']:.1f} solves/s, p50 {s['p50_ms']:.0f} ms, "
         f"p99 {s['p99_ms']:.0f} ms in {time.perf_counter() - t0:.1f} s")
+
+    def slim(o: dict) -> dict:
+        keep = ("solves", "solves_per_s", "p50_ms", "p99_ms", "max_ms")
+        return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in o.items() if k in keep}
+
     batch = {"workload": b["workload"],
-             "batched_solves_per_s": round(b["batched"]["solves_per_s"], 2),
-             "per_solve_solves_per_s": round(b["per_solve"]["solves_per_s"], 3),
-             "per_solve_sample": b["per_solve"]["sample"],
+             "batched": {**slim(bb), "effective_glups": round(bb["effective_glups"], 1),
+                         **{g: slim(bb[g]) for g in ("reached_target", "stagnated", "max_sweeps")
+                            if g in bb}},
+             "per_solve": {**slim(ps), "sample": ps["solves"], "rung": ps["rung"],
+                           **{g: slim(ps[g]) for g in ("reached_target", "stagnated", "max_sweeps")
+                              if g in ps}},
              "speedup": round(b["speedup"], 2),
+             "speedup_reached_target": round(b.get("speedup_reached_target", float("nan")), 2),
              "per_solve_graphs_solves_per_s": round(b["per_solve_graphs"]["solves_per_s"], 3),
              "speedup_vs_graphs": round(b["speedup_vs_graphs"], 2),
-             "jacobi": b["jacobi"],
+             "jacobi": bb["jacobi"],
              "baseline_agrees_on_steps_and_status": b["baseline_agrees_on_steps_and_status"],
+             "accuracy_vs_fp64": b["accuracy_vs_fp64"],
              "cpu_baseline": cpu}
-    serve = {"workload": s["workload"], "solves_per_s": round(s["solves_per_s"], 2),
-             "p50_ms": round(s["p50_ms"], 1), "p99_ms": round(s["p99_ms"], 1),
-             "statuses": s["statuses"]}
+    serve = {"workload": s["workload"], **slim(s),
+             **{g: slim(s[g]) for g in ("reached_target", "stagnated", "max_sweeps") if g in s},
+             "statuses": s["statuses"], "accuracy_vs_fp64": s["accuracy_vs_fp64"]}
     return batch, serve
 
 

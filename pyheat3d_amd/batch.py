@@ -245,6 +245,9 @@ class BatchSolver:
     def run(self, max_rounds: int = 10 ** 6) -> dict:
         """Replay rounds until every solve has finished; returns timing and round counts.
 
+        Every request counts as submitted when the run starts; its completion time is the end
+        of the round in which it finished (``finish_s``, seconds from the start).
+
         When half of the table has finished, the finished solves are dropped from the device
         table (their state is kept on the host) and the round graph is re-recorded, so the
         passes of the remaining solves do not walk the tiles of finished ones.
@@ -252,6 +255,7 @@ class BatchSolver:
         graph = None
         t0 = time.perf_counter()
         rounds = compactions = 0
+        self.finish_s: list[float | None] = [None] * len(self.requests)
         try:
             while rounds < max_rounds:
                 if graph is None and self.use_graph:
@@ -262,8 +266,11 @@ class BatchSolver:
                     self.batch.round(self.round_sweeps)
                 rounds += 1
                 descs = self.batch.get()
+                t_round = time.perf_counter() - t0
                 for row, d in zip(self.table, descs):
                     self.state[row] = d
+                    if d.status != 0 and self.finish_s[row] is None:
+                        self.finish_s[row] = t_round
                 running = [row for row, d in zip(self.table, descs) if d.status == 0]
                 if not running:
                     break
@@ -281,11 +288,15 @@ class BatchSolver:
                 graph.close()
         return {"seconds": time.perf_counter() - t0, "rounds": rounds, "compactions": compactions}
 
-    def results(self, keep_fields: bool = False) -> list[Result]:
+    def results(self, keep_fields: bool | set[int] = False) -> list[Result]:
+        """One result per request; ``keep_fields`` downloads every final field, or the fields
+        of the request ids in a set."""
         out = []
-        for req, f, d in zip(self.requests, self.fields, self.state):
+        finish = getattr(self, "finish_s", [None] * len(self.requests))
+        for req, f, d, t in zip(self.requests, self.fields, self.state, finish):
             res = _result(req, d)
-            if keep_fields:
+            res.seconds = t
+            if keep_fields is True or (keep_fields and req.rid in keep_fields):
                 res.u = f.download(current_buffer(d), self.stream)
             out.append(res)
         return out
@@ -301,11 +312,17 @@ class BatchSolver:
 
 
 def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
-                     check_every: int = _hip.BATCH_CHECK,
-                     graphs: bool = False) -> tuple[list[Result], float]:
-    """Baseline: each request on its own stepper, one after another; one launch per sweep (the
-    reference's stepping), or with ``graphs`` the stepper's replayed graphs of 16 sweeps for
+                     check_every: int = _hip.BATCH_CHECK, graphs: bool = False,
+                     kpass: dict[int, int] | None = None) -> tuple[list[Result], float]:
+    """Baseline: each request on its own stepper, one after another; one launch per pass (the
+    reference's stepping), or with ``graphs`` the stepper's replayed graphs of 16 passes for
     grids of up to ``HipStepper.GRAPH_MAX_SITES`` sites.
+
+    ``kpass`` (request id -> steps fused per pass, as the batch chose them: the ``kpass`` field
+    of its solve table) makes each solve run the same rung as in the batch, the k-step
+    temporal-blocking kernel (same float32 bits as k single steps), so that a batched-vs-per-
+    solve speedup measures batching, not temporal blocking. Without it every solve runs the
+    single-step march rung (z-march in 3D, y-march in 2D).
 
     Jacobi requests use the same residual definition, check cadence, tolerance, stagnation test
     and cap as the batch (:func:`pyheat3d_amd.jacobi.solve`).
@@ -316,8 +333,11 @@ def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
     t_all = time.perf_counter()
     out = []
     for req in requests:
-        # the single-grid march rungs: z-march in 3D, y-march in 2D
-        cls = make("zmarch", req.storage) if req.dims == 3 else make("tb", req.storage, 1)
+        k = (kpass or {}).get(req.rid, 1)
+        if k > 1:
+            cls = make("tb", req.storage, k)
+        else:   # the single-grid march rungs: z-march in 3D, y-march in 2D
+            cls = make("zmarch", req.storage) if req.dims == 3 else make("tb", req.storage, 1)
         stream = _hip.Stream()
         f = DeviceFields(req, stream)
         stream.sync()
@@ -346,3 +366,25 @@ def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
         res.seconds = time.perf_counter() - t0
         out.append(res)
     return out, time.perf_counter() - t_all
+
+
+def batch_kpass(solver: BatchSolver) -> dict[int, int]:
+    """Steps fused per pass that the batch chose for each request (its solve table)."""
+    return {req.rid: max(1, int(d.kpass)) for req, d in zip(solver.requests, solver.state)}
+
+
+def error_vs_fp64(req: Request, u: np.ndarray) -> float:
+    """Max-norm relative error of a final field against the FP64 NumPy solution of the same
+    request (:mod:`pyheat3d_amd.fp64`): heat, the same steps in float64; Jacobi, the fixed point
+    of the device's Jacobi map solved exactly in float64."""
+    from . import fp64
+    from .analytic import splitmix_field
+
+    u0 = splitmix_field(req.shape, req.seed, req.kind)
+    st = req.stencil()
+    if req.mode == "heat":
+        ref = fp64.heat(u0, st.c0, st.c1, req.bc, req.steps)
+    else:
+        ref = fp64.dirichlet_solution(u0, st.c1, req.source)
+    return fp64.rel_err(u, ref)
+

@@ -161,78 +161,165 @@ def run(mode: str = "quick", out_dir: str | Path = "results", variants: list[str
 # batched and served solves
 # ----------------------------------------------------------------------------------------------
 
-def _lat_stats(ms: np.ndarray) -> dict[str, float]:
+def _lat_stats(ms) -> dict[str, float | None]:
+    ms = np.asarray(ms, dtype=np.float64)
+    if ms.size == 0:
+        return {"p50_ms": None, "p99_ms": None, "max_ms": None}
     return {"p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
             "max_ms": float(ms.max())}
 
 
-def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 512),
-                    n3: tuple[int, int] = (32, 128), frac_jacobi: float = 0.5,
-                    storage: str = "float32", baseline_every: int = 8, round_sweeps: int = 256,
-                    jacobi_cap: int = 10 ** 6) -> dict[str, Any]:
-    """All requests submitted at t = 0: batched graph rounds vs one launch per sweep per solve.
+def reached(req, status: str) -> bool:
+    """A heat solve that ran its steps, or a Jacobi solve whose FP64 relative residual reached
+    its tolerance (not one that stagnated at the storage floor or hit the sweep cap)."""
+    return status == "done"
 
-    The per-solve baseline runs every ``baseline_every``-th request (a systematic sample of the
-    same mix); its solves/s is the sample size over the sum of its solve times. The same sample
-    is also solved with the single-grid stepper's replayed graphs (``per_solve_graphs``), the
-    stronger per-solve path for small grids.
+
+def outcome_stats(reqs, results, seconds: float) -> dict[str, Any]:
+    """Solves/s and completion-latency percentiles, over every solve and per outcome: reached
+    the target (heat done, Jacobi at tol), stagnated at the storage floor, capped."""
+    groups: dict[str, list] = {"reached_target": [], "stagnated": [], "max_sweeps": []}
+    for q, r in zip(reqs, results):
+        key = "reached_target" if reached(q, r.status) else r.status
+        groups.setdefault(key, []).append(r)
+    out: dict[str, Any] = {"solves": len(results), "seconds": seconds,
+                           "solves_per_s": len(results) / seconds,
+                           **_lat_stats([r.seconds * 1e3 for r in results])}
+    for key, rs in groups.items():
+        out[key] = {"solves": len(rs), "solves_per_s": len(rs) / seconds,
+                    **_lat_stats([r.seconds * 1e3 for r in rs])}
+    jac = [(q, r) for q, r in zip(reqs, results) if q.mode == "jacobi"]
+    out["jacobi"] = {"solves": len(jac),
+                     "converged": sum(r.status == "done" for _, r in jac),
+                     "stagnated": sum(r.status == "stagnated" for _, r in jac),
+                     "capped": sum(r.status == "max_sweeps" for _, r in jac),
+                     "max_rel_residual": max((r.rel_residual or 0.0) for _, r in jac) if jac else None}
+    return out
+
+
+def accuracy_sample(reqs, max_sites: int = 1 << 22, max_heat_work: float = 2e9,
+                    every: int = 8) -> set[int]:
+    """Request ids whose error against the FP64 solution is measured: every ``every``-th request
+    whose FP64 solution costs seconds on one core (Jacobi: at most ``max_sites`` sites, one DST
+    solve; heat: at most ``max_heat_work`` site-steps in float64)."""
+    out = set()
+    for q in reqs[::every]:
+        if q.mode == "heat" and q.sites * q.steps <= max_heat_work:
+            out.add(q.rid)
+        elif q.mode == "jacobi" and q.sites <= max_sites:
+            out.add(q.rid)
+    return out
+
+
+def accuracy_stats(reqs, results) -> dict[str, Any]:
+    """Max and median error vs FP64 per (mode, outcome) over the results that carry a field."""
+    from .batch import error_vs_fp64
+
+    rows = []
+    for q, r in zip(reqs, results):
+        if r.u is None:
+            continue
+        e = error_vs_fp64(q, r.u)
+        r.extra["err_vs_fp64"] = e
+        r.u = None
+        rows.append((q.mode, "reached_target" if reached(q, r.status) else r.status, e))
+    out: dict[str, Any] = {"measured": len(rows)}
+    for mode in ("heat", "jacobi"):
+        for oc in ("reached_target", "stagnated", "max_sweeps"):
+            es = [e for m, o, e in rows if m == mode and o == oc]
+            if es:
+                out[f"{mode}/{oc}"] = {"n": len(es), "max": float(max(es)),
+                                       "median": float(np.median(es))}
+    return out
+
+
+def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 2048),
+                    n3: tuple[int, int] = (32, 256), frac_jacobi: float = 0.5,
+                    storage: str = "float32", baseline_every: int = 8, round_sweeps: int = 256,
+                    jacobi_cap: int = 10 ** 6, accuracy_every: int = 8,
+                    graphs_baseline: bool = True) -> dict[str, Any]:
+    """All requests submitted at t = 0: batched graph rounds vs one launch per pass per solve.
+
+    The per-solve leg runs every ``baseline_every``-th request (a systematic sample of the same
+    mix), each with the fused-step rung the batch chose for it (the same float32 bits), so the
+    speedup measures batching alone; its solves/s is the sample size over the sum of its solve
+    times. With ``graphs_baseline`` the sample also runs on the single-grid stepper's replayed
+    graphs (the stronger per-solve path for small grids). Solves/s and completion latency are
+    reported over all solves and separately for those that reached their target (heat: all
+    steps; Jacobi: the FP64 relative residual at ``tol``), stagnated at the storage floor or hit
+    the sweep cap; a sample of final fields is compared with FP64 NumPy solutions.
     """
-    from .batch import BatchSolver, make_mix, solve_one_by_one
+    from .batch import BatchSolver, batch_kpass, make_mix, solve_one_by_one
 
     reqs = make_mix(n, seed, n2=n2, n3=n3, frac_jacobi=frac_jacobi, storage=storage,
                     jacobi_cap=jacobi_cap)
+    acc_ids = accuracy_sample(reqs, every=accuracy_every)
     bs = BatchSolver(reqs, round_sweeps=round_sweeps)
     try:
         timing = bs.run()
-        res = bs.results()
+        kp = batch_kpass(bs)
+        res = bs.results(keep_fields=acc_ids)
     finally:
         bs.close()
+    work = sum(q.sites * r.steps for q, r in zip(reqs, res))
     sample = reqs[::baseline_every]
-    base, _ = solve_one_by_one(sample)
+    base, _ = solve_one_by_one(sample, kpass=kp)
     base_time = sum(r.seconds for r in base)
-    based, _ = solve_one_by_one(sample, graphs=True)
-    based_time = sum(r.seconds for r in based)
-    jac = [r for q, r in zip(reqs, res) if q.mode == "jacobi"]
     by_rid = {r.rid: r for r in res}
     agree = sum(1 for b in base if by_rid[b.rid].steps == b.steps and by_rid[b.rid].status == b.status)
-    return {
-        "workload": f"{n} solves, {int(100 * (1 - frac_jacobi))}% heat (200 steps) / "
-                    f"{int(100 * frac_jacobi)}% Jacobi Laplace to 1e-6, 60% 2D N in {list(n2)}, "
-                    f"40% 3D N in {list(n3)}, {storage}, seed {seed}",
-        "batched": {"seconds": timing["seconds"], "rounds": timing["rounds"],
-                    "solves_per_s": n / timing["seconds"]},
-        "per_solve": {"sample": len(sample), "seconds": base_time,
-                      "solves_per_s": len(sample) / base_time,
-                      **_lat_stats(np.array([r.seconds for r in base]) * 1e3)},
-        "per_solve_graphs": {"seconds": based_time, "solves_per_s": len(sample) / based_time},
+    out: dict[str, Any] = {
+        "workload": f"{n} solves, {int(round(100 * (1 - frac_jacobi)))}% heat (200 steps) / "
+                    f"{int(round(100 * frac_jacobi))}% Jacobi Laplace to 1e-6 (sweep cap "
+                    f"{jacobi_cap}), 60% 2D N in {list(n2)}, 40% 3D N in {list(n3)}, {storage}, "
+                    f"seed {seed}",
+        "batched": {"rounds": timing["rounds"], "site_updates": work,
+                    "effective_glups": work / timing["seconds"] / 1e9,
+                    **outcome_stats(reqs, res, timing["seconds"])},
+        "per_solve": {"sample": len(sample), "every": baseline_every,
+                      "rung": "the batch's fused-step rung per solve (tb<kpass>, or the march)",
+                      **outcome_stats(sample, base, base_time)},
         "speedup": (n / timing["seconds"]) / (len(sample) / base_time),
-        "speedup_vs_graphs": (n / timing["seconds"]) / (len(sample) / based_time),
-        "jacobi": {"converged": sum(r.status == "done" for r in jac),
-                   "stagnated": sum(r.status == "stagnated" for r in jac),
-                   "capped": sum(r.status == "max_sweeps" for r in jac),
-                   "max_rel_residual": max((r.rel_residual or 0.0) for r in jac) if jac else None},
         "baseline_agrees_on_steps_and_status": f"{agree}/{len(sample)}",
         "sample_sweeps": {int(b.rid): int(by_rid[b.rid].steps) for b in base},
+        "accuracy_vs_fp64": accuracy_stats(reqs, res),
     }
+    rs = out["batched"]["reached_target"]
+    ps = out["per_solve"]["reached_target"]
+    if ps["solves"] and rs["solves"]:
+        out["speedup_reached_target"] = rs["solves_per_s"] / ps["solves_per_s"]
+    if graphs_baseline:
+        based, _ = solve_one_by_one(sample, graphs=True, kpass=kp)
+        based_time = sum(r.seconds for r in based)
+        out["per_solve_graphs"] = {"seconds": based_time, "solves_per_s": len(sample) / based_time}
+        out["speedup_vs_graphs"] = (n / timing["seconds"]) / (len(sample) / based_time)
+    return out
 
 
 def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
-                    n2: tuple[int, int] = (64, 512), n3: tuple[int, int] = (32, 128),
+                    n2: tuple[int, int] = (64, 2048), n3: tuple[int, int] = (32, 256),
                     storage: str = "float32", budget_gb: float = 200.0,
-                    round_sweeps: int = 128) -> dict[str, Any]:
+                    round_sweeps: int = 128, jacobi_cap: int = 10 ** 6,
+                    accuracy_every: int = 8, time_limit: float = 3600.0) -> dict[str, Any]:
+    """The mix arriving as a Poisson process of ``rate`` requests/s, served by continuous
+    batching under an HBM budget; latency = completion - arrival, reported over all solves and
+    per outcome, with a sample of results compared with FP64 NumPy solutions."""
     from .batch import make_mix
     from .server import Server, poisson_arrivals
 
-    reqs = make_mix(n, seed, n2=n2, n3=n3, storage=storage)
+    reqs = make_mix(n, seed, n2=n2, n3=n3, storage=storage, jacobi_cap=jacobi_cap)
     arr = poisson_arrivals(n, rate, seed + 1)
-    srv = Server(int(budget_gb * 1e9), round_sweeps=round_sweeps)
+    acc_ids = accuracy_sample(reqs, every=accuracy_every)
+    srv = Server(int(budget_gb * 1e9), round_sweeps=round_sweeps, keep_fields=acc_ids)
     try:
-        res, st = srv.serve(reqs, arr)
+        res, st = srv.serve(reqs, arr, time_limit=time_limit)
     finally:
         srv.close()
-    return {"workload": f"{n} Poisson arrivals at {rate:g}/s, mix as the batch benchmark, "
-                        f"{storage}, HBM budget {budget_gb:g} GB",
-            "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
-            "max_ms": st.max_ms, "seconds": st.seconds, "rounds": st.rounds,
-            "max_resident": st.max_resident, "max_resident_GB": st.max_resident_bytes / 1e9,
-            "statuses": {s: sum(r.status == s for r in res) for s in ("done", "stagnated", "max_sweeps")}}
+    return {"workload": f"{n} Poisson arrivals at {rate:g}/s of the mix: 50% heat (200 steps) / "
+                        f"50% Jacobi Laplace to 1e-6 (sweep cap {jacobi_cap}), 60% 2D N in "
+                        f"{list(n2)}, 40% 3D N in {list(n3)}, {storage}, seed {seed}, HBM budget "
+                        f"{budget_gb:g} GB",
+            "rounds": st.rounds, "max_resident": st.max_resident,
+            "max_resident_GB": st.max_resident_bytes / 1e9,
+            **outcome_stats(reqs, res, st.seconds),
+            "statuses": {s_: sum(r.status == s_ for r in res) for s_ in ("done", "stagnated", "max_sweeps")},
+            "accuracy_vs_fp64": accuracy_stats(reqs, res)}

@@ -21,41 +21,64 @@ from .registry import VARIANTS, availability, make_stepper
 
 
 def _cmd_list(_: argparse.Namespace) -> int:
-    print(f"{'variant':30s} {'dims':6s} {'storage':9s} {'k':>2s}  status")
-    for key, v in VARIANTS.items():
+    """Variants grouped by rung, with what each needs and whether this machine can run it."""
+    rows = []
+    for key, v in sorted(VARIANTS.items(), key=lambda kv: (kv[1].rung, kv[0])):
         ok, why = availability(key)
-        dims = ",".join(str(d) for d in v.dims)
-        print(f"{key:30s} {dims:6s} {v.storage:9s} {v.steps_per_pass:>2d}  "
-              f"{'ok' if ok else 'unavailable (' + why + ')'}")
+        rows.append((str(v.rung), key, "/".join(f"{d}D" for d in v.dims), v.storage,
+                     str(v.steps_per_pass), "yes" if v.source else "no",
+                     "ready" if ok else f"unavailable: {why}"))
+    head = ("rung", "key", "dims", "storage", "k", "source", "here")
+    width = [max(len(r[c]) for r in rows + [head]) for c in range(len(head) - 1)]
+    for r in [head] + rows:
+        print("  ".join(x.ljust(w) for x, w in zip(r, width)) + "  " + r[-1])
     return 0
 
 
+def _initial_state(a: argparse.Namespace):
+    """Stencil and float32 initial field of a ``run`` invocation, plus the per-step decay of
+    the sine mode when that is the initial state (for the closed-form check)."""
+    shape = (a.n,) * a.dims
+    if a.mode == "jacobi":
+        u0 = an.harmonic_sinh(a.n, a.dims).astype(np.float32)
+        u0[(slice(1, -1),) * a.dims] = 0.0      # Laplace: boundary data, zero guess
+        return jacobi_stencil(a.dims, a.bc), u0, None
+    r = a.r if a.r is not None else 0.9 * stability_limit(a.dims)
+    if a.init == "random":
+        return heat_stencil(a.dims, r, a.bc), np.random.default_rng(0).random(shape).astype(np.float32), None
+    g = an.ftcs_amplification(r, a.n, a.dims, a.bc)
+    return heat_stencil(a.dims, r, a.bc), an.fourier_mode(a.n, a.dims, a.bc).astype(np.float32), g
+
+
 def _cmd_run(a: argparse.Namespace) -> int:
-    dims = a.dims
-    shape = (a.n,) * dims
-    if a.mode == "heat":
-        r = a.r if a.r is not None else 0.9 * stability_limit(dims)
-        st = heat_stencil(dims, r, a.bc)
-        u0 = (an.fourier_mode(a.n, dims, a.bc) if a.init == "mode"
-              else np.random.default_rng(0).random(shape)).astype(np.float32)
-    else:
-        st = jacobi_stencil(dims, a.bc)
-        u0 = an.harmonic_sinh(a.n, dims).astype(np.float32)
-        u0[(slice(1, -1),) * dims] = 0.0
-    s = make_stepper(a.variant, u0, st)
+    """One timed run of one variant; with the sine-mode initial state the final amplitude is
+    checked against the closed form g^steps."""
+    stencil, u0, g = _initial_state(a)
+    st = make_stepper(a.variant, u0, stencil)
     try:
-        s.step(s.steps_per_pass)
-        s.synchronize()
+        st.step(st.steps_per_pass)           # compile-free warm-up: one pass
+        st.synchronize()
         t0 = time.perf_counter()
-        s.step(a.steps)
-        s.synchronize()
+        st.step(a.steps)
+        st.synchronize()
         dt = time.perf_counter() - t0
-        u = s.get()
+        u = st.get()
+        sites = st.ncells
+        k = st.steps_per_pass
     finally:
-        s.close()
-    print(f"{a.variant}: {dims}D n={a.n} {a.mode} {a.bc}, {a.steps} steps in {dt:.3f} s -> "
-          f"{s.ncells * a.steps / dt / 1e9:.2f} GLUPS (development timing, not a benchmark)")
-    print(f"state: min {u.min():.4g} max {u.max():.4g} mean {u.mean():.4g}")
+        st.close()
+    report = {"variant": a.variant, "shape": list(u0.shape), "bc": a.bc, "mode": a.mode,
+              "steps_timed": a.steps, "steps_per_pass": k, "seconds": round(dt, 6),
+              "GLUPS": round(sites * a.steps / dt / 1e9, 3),
+              "field": {"min": float(u.min()), "max": float(u.max()), "mean": float(u.mean())}}
+    if g is not None:
+        total = a.steps + k
+        ref = u0.astype(np.float64) * g ** total
+        report["mode_check"] = {"steps": total, "closed_form_amplitude": g ** total,
+                                "max_rel_dev": float(np.abs(u - ref).max() / max(np.abs(ref).max(), 1e-300))}
+    for key, val in report.items():
+        print(f"{key:>14}: {val}")
+    print("(one run after one warm-up pass: a quick check, not the benchmark protocol; see bench)")
     return 0
 
 
@@ -95,7 +118,8 @@ def _cmd_batch(a: argparse.Namespace) -> int:
     from .benchmark import batch_benchmark
 
     r = batch_benchmark(a.n, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
-                        a.frac_jacobi, a.storage, a.baseline_every, a.round_sweeps)
+                        a.frac_jacobi, a.storage, a.baseline_every, a.round_sweeps,
+                        a.jacobi_cap, a.accuracy_every)
     print(json.dumps(r, indent=1))
     return 0
 
@@ -104,7 +128,7 @@ def _cmd_serve(a: argparse.Namespace) -> int:
     from .benchmark import serve_benchmark
 
     r = serve_benchmark(a.n, a.rate, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
-                        a.storage, a.budget_gb, a.round_sweeps)
+                        a.storage, a.budget_gb, a.round_sweeps, a.jacobi_cap, a.accuracy_every)
     print(json.dumps(r, indent=1))
     return 0
 
@@ -141,9 +165,12 @@ def _cmd_tune(a: argparse.Namespace) -> int:
 
 def _size_args(p: argparse.ArgumentParser) -> None:
     p.add_argument("--n2-min", type=int, default=64)
-    p.add_argument("--n2-max", type=int, default=512)
+    p.add_argument("--n2-max", type=int, default=2048)
     p.add_argument("--n3-min", type=int, default=32)
-    p.add_argument("--n3-max", type=int, default=128)
+    p.add_argument("--n3-max", type=int, default=256)
+    p.add_argument("--jacobi-cap", type=int, default=10 ** 6, help="sweep cap of Jacobi solves")
+    p.add_argument("--accuracy-every", type=int, default=8,
+                   help="compare every k-th result with its FP64 NumPy solution (bounded size)")
     p.add_argument("--storage", default="float32", choices=("float32", "float16", "bfloat16"))
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--round-sweeps", type=int, default=256)

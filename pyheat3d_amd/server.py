@@ -54,7 +54,9 @@ class Server:
     """Admission by HBM budget, resident fields, round-based continuous batching."""
 
     def __init__(self, hbm_budget_bytes: int = 200 * 10 ** 9, round_sweeps: int = 128,
-                 capacity: int = 4096, use_graph: bool = True, keep_fields: bool = False):
+                 capacity: int = 4096, use_graph: bool = True,
+                 keep_fields: bool | set[int] = False):
+        """``keep_fields``: download every final field, or those of the request ids in a set."""
         self.budget = hbm_budget_bytes
         self.round_sweeps = round_sweeps
         self.use_graph = use_graph
@@ -135,7 +137,8 @@ class Server:
                         rel = math.sqrt(d.res_sq / d.res0_sq) if d.res0_sq > 0 else 0.0
                     res = Result(req.rid, STATUS[d.status], int(d.done), rel,
                                  seconds=t_done - arr)
-                    if self.keep_fields:
+                    if self.keep_fields is True or (self.keep_fields and
+                                                    req.rid in self.keep_fields):
                         res.u = f.download(current_buffer(d), self.stream)
                     f.close()
                     used -= req.device_bytes()

@@ -125,3 +125,49 @@ def test_jacobi_converges_on_small_grid():
     assert r.status == "done" and r.rel_residual <= 1e-6
     predicted = np.log(1e-6) / np.log(np.cos(np.pi / 16))
     assert r.steps < 3 * predicted
+
+
+def test_per_solve_on_the_batch_rung_equals_the_batch():
+    """The comparator that runs each solve on the fused-step rung the batch chose (tb<kpass>)
+    gives the batch's bits, step counts and statuses, and that rung is really fused (kpass > 1
+    for the float32 tier-2 solves of the mix)."""
+    from pyheat3d_amd.batch import batch_kpass
+
+    reqs = mix()
+    b = BatchSolver(reqs, round_sweeps=128)
+    try:
+        b.run()
+        kp = batch_kpass(b)
+        batched = b.results(keep_fields=True)
+    finally:
+        b.close()
+    assert any(k > 1 for k in kp.values()), kp
+    per, _ = solve_one_by_one(reqs, keep_fields=True, kpass=kp)
+    for req, x, y in zip(reqs, batched, per):
+        assert (x.status, x.steps) == (y.status, y.steps), (req.shape, req.mode, kp[req.rid])
+        assert np.array_equal(x.u, y.u), (req.shape, req.mode, kp[req.rid])
+
+
+def test_results_against_fp64_solutions():
+    """Heat results within the reference's float32 tolerance of the FP64 replay; a Jacobi solve
+    that converged to 1e-6 within 1e-5 of the exact FP64 fixed point; completion times set."""
+    from pyheat3d_amd.batch import error_vs_fp64
+
+    reqs = [Request(shape=(130, 150), mode="heat", steps=200, seed=41, rid=0),
+            Request(shape=(40, 36, 30), mode="heat", steps=200, seed=42, rid=1),
+            Request(shape=(17, 17), mode="jacobi", steps=100000, seed=43, rid=2),
+            Request(shape=(12, 13, 14), mode="jacobi", steps=100000, seed=44, rid=3)]
+    b = BatchSolver(reqs, round_sweeps=256)
+    try:
+        b.run()
+        res = b.results(keep_fields={0, 1, 2, 3})
+    finally:
+        b.close()
+    for q, r in zip(reqs, res):
+        assert r.seconds is not None and r.seconds > 0
+        e = error_vs_fp64(q, r.u)
+        print(f"[gpu] {q.shape} {q.mode} {r.status} error vs FP64 {e:.3e}", flush=True)
+        if q.mode == "heat":
+            assert e <= nr.TOLERANCE["float32"], (q.shape, e)
+        else:
+            assert r.status == "done" and e <= 1e-5, (q.shape, r.status, e)
