@@ -180,7 +180,13 @@ typedef struct ph3_solve_desc {
     int kpass;                   /* steps fused per tier-2 pass: 1, or for float32 storage 2 / 4
                                     / 8 dividing PH3_BATCH_CHECK and `steps` (0 means 1). The
                                     state lives in buf[(done / kpass) & 1]. */
-    int reserved;                /* must be 0 */
+    int tile;                    /* tier-2 tile shape of fused passes: 0 = the library's rule;
+                                    3D 1: 1-site lanes, 32 rows; 2: 2-site lanes, 32 rows;
+                                    3: 1-site lanes, 64 rows; 4: 2-site lanes, 32 rows in 8
+                                    waves. 2D 1/2: 1/2-site lanes; 3/4: the same with a store
+                                    on every step. 2-site shapes need an even width above 64
+                                    and 8-byte aligned buffers (else the 1-site shape). Never
+                                    changes the result. */
     long long steps;             /* mode 0: steps; mode 1: sweep cap */
     double tol;                  /* mode 1: target ||r|| / ||r0|| */
     double log_rate;             /* mode 1: ln(Jacobi spectral radius) * PH3_BATCH_CHECK (< 0) */
@@ -199,6 +205,12 @@ typedef struct ph3_batch ph3_batch;
 size_t ph3_solve_desc_bytes(void);  /* sizeof(ph3_solve_desc), to check foreign mirrors */
 int ph3_batch_create(ph3_batch** batch, int capacity);
 int ph3_batch_destroy(ph3_batch* batch);
+/* March-chunk policy of the tier-2 classes of one (dims, kpass), taking effect at the next
+ * ph3_batch_set: a class's grids are cut into about target_tiles tiles together (each grid into
+ * at most 512), with chunks of at most max_chunk planes / rows. 0 restores the default (512
+ * tiles per grid, chunks up to 64, or 128 for 8-step passes). Never changes the result (the
+ * autotuner sets it: the reference's bench/tune.py:21-95 tunes single-grid tiles only). */
+int ph3_batch_params(ph3_batch* batch, int dims, int kpass, int target_tiles, int max_chunk);
 /* Replace the batch's solve table with descs[0..n) (host memory) and compute the initial
  * residual of every mode-1 solve whose `checks` is 0. Synchronous. */
 int ph3_batch_set(ph3_batch* batch, const ph3_solve_desc* descs, int n, void* stream);

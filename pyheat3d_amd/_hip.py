@@ -79,6 +79,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ph3_solve_desc_bytes": (c_size_t, []),
     "ph3_batch_create": (c_int, [POINTER(c_void_p), c_int]),
     "ph3_batch_destroy": (c_int, [c_void_p]),
+    "ph3_batch_params": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "ph3_batch_set": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "ph3_batch_round": (c_int, [c_void_p, c_int, c_void_p]),
     "ph3_batch_get": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
@@ -96,7 +97,7 @@ class SolveDesc(ctypes.Structure):
         ("dims", c_int), ("nz", c_int), ("ny", c_int), ("nx", c_int),
         ("periodic", c_int), ("storage", c_int), ("c0", c_float), ("c1", c_float),
         ("buf", c_void_p * 2), ("src", c_void_p), ("mode", c_int), ("stall_window", c_int),
-        ("kpass", c_int), ("reserved", c_int), ("steps", ctypes.c_longlong), ("tol", c_double), ("log_rate", c_double),
+        ("kpass", c_int), ("tile", c_int), ("steps", ctypes.c_longlong), ("tol", c_double), ("log_rate", c_double),
         ("stall_fraction", c_double), ("done", ctypes.c_longlong), ("res0_sq", c_double),
         ("res_sq", c_double), ("status", c_int), ("checks", c_int),
         ("ring", c_double * STALL_RING),

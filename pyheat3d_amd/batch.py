@@ -47,6 +47,7 @@ class Request:
     source: np.ndarray | None = None   # optional float32 source (Poisson)
     rid: int = 0
     bc: str = "dirichlet"         # "dirichlet" | "periodic" (heat only)
+    tile: int = 0                 # tier-2 tile shape of fused passes (0: the library's rule)
 
     def __post_init__(self) -> None:
         if self.bc not in ("dirichlet", "periodic"):
@@ -153,6 +154,22 @@ class DeviceFields:
         self.buf, self.src = [0, 0], 0
 
 
+def apply_tuned_tiles(requests: list[Request]) -> None:
+    """Give every request without a tile shape the tuned one for its dims and size."""
+    from .tune import batch_tile
+
+    for r in requests:
+        if r.tile == 0:
+            r.tile = batch_tile(r.dims, r.shape)
+
+
+def default_params() -> list[tuple[int, int, int, int]]:
+    """The tuned chunk policies of the batch classes (none when the table holds none)."""
+    from .tune import batch_policy
+
+    return batch_policy()
+
+
 def make_desc(req: Request, f: DeviceFields) -> _hip.SolveDesc:
     st = req.stencil()
     d = _hip.SolveDesc()
@@ -164,6 +181,7 @@ def make_desc(req: Request, f: DeviceFields) -> _hip.SolveDesc:
     d.buf[0], d.buf[1] = f.buf
     d.src = f.src or None
     d.mode = 0 if req.mode == "heat" else 1
+    d.tile = req.tile
     d.steps = req.steps
     d.tol = req.tol
     d.stall_window = req.stall_window if req.mode == "jacobi" else 0
@@ -183,6 +201,11 @@ class Batch:
         self.handle = int(p.value or 0)
         self.stream = stream
         self.n = 0
+
+    def params(self, dims: int, kpass: int, target_tiles: int, max_chunk: int) -> None:
+        """March-chunk policy of the (dims, kpass) classes from the next :meth:`set` on."""
+        _hip.check(_hip.lib().ph3_batch_params(self.handle, dims, kpass, target_tiles,
+                                               max_chunk), "batch_params")
 
     def set(self, descs: list[_hip.SolveDesc]) -> None:
         arr = (_hip.SolveDesc * max(1, len(descs)))(*descs)
@@ -222,7 +245,11 @@ class BatchSolver:
     """All requests resident at once, advanced by graph-replayed batched rounds."""
 
     def __init__(self, requests: list[Request], round_sweeps: int = 256, use_graph: bool = True,
-                 compact: bool = True):
+                 compact: bool = True, params: list[tuple[int, int, int, int]] | None = None,
+                 tuned_tiles: bool = True):
+        """``params``: (dims, kpass, target_tiles, max_chunk) chunk policies, see
+        :meth:`Batch.params` (default: the tuning table's, :func:`pyheat3d_amd.tune.batch_policy`).
+        ``tuned_tiles``: requests without a tile shape get the table's (else the library's rule)."""
         self.requests = requests
         self.round_sweeps = round_sweeps
         self.use_graph = use_graph
@@ -230,11 +257,15 @@ class BatchSolver:
         self.stream = _hip.Stream()
         self.fields: list[DeviceFields] = []
         self.batch: Batch | None = None
+        if tuned_tiles:
+            apply_tuned_tiles(requests)
         try:
             for r in requests:
                 self.fields.append(DeviceFields(r, self.stream))
             self.stream.sync()
             self.batch = Batch(len(requests), self.stream)
+            for pr in (params if params is not None else default_params()):
+                self.batch.params(*pr)
             self.state = [make_desc(r, f) for r, f in zip(requests, self.fields)]
             self.table = list(range(len(requests)))  # request index of each table row
             self.batch.set(self.state)

@@ -237,7 +237,7 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
                     n3: tuple[int, int] = (32, 256), frac_jacobi: float = 0.5,
                     storage: str = "float32", baseline_every: int = 8, round_sweeps: int = 256,
                     jacobi_cap: int = 10 ** 6, accuracy_every: int = 8,
-                    graphs_baseline: bool = True) -> dict[str, Any]:
+                    graphs_baseline: bool = True, use_graph: bool = True) -> dict[str, Any]:
     """All requests submitted at t = 0: batched graph rounds vs one launch per pass per solve.
 
     The per-solve leg runs every ``baseline_every``-th request (a systematic sample of the same
@@ -248,13 +248,15 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
     reported over all solves and separately for those that reached their target (heat: all
     steps; Jacobi: the FP64 relative residual at ``tol``), stagnated at the storage floor or hit
     the sweep cap; a sample of final fields is compared with FP64 NumPy solutions.
+    ``use_graph=False`` enqueues every round eagerly instead of replaying its captured HIP graph
+    (for kernel traces: see the known limits of the batch).
     """
     from .batch import BatchSolver, batch_kpass, make_mix, solve_one_by_one
 
     reqs = make_mix(n, seed, n2=n2, n3=n3, frac_jacobi=frac_jacobi, storage=storage,
                     jacobi_cap=jacobi_cap)
     acc_ids = accuracy_sample(reqs, every=accuracy_every)
-    bs = BatchSolver(reqs, round_sweeps=round_sweeps)
+    bs = BatchSolver(reqs, round_sweeps=round_sweeps, use_graph=use_graph)
     try:
         timing = bs.run()
         kp = batch_kpass(bs)
@@ -299,7 +301,8 @@ def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
                     n2: tuple[int, int] = (64, 2048), n3: tuple[int, int] = (32, 256),
                     storage: str = "float32", budget_gb: float = 200.0,
                     round_sweeps: int = 128, jacobi_cap: int = 10 ** 6,
-                    accuracy_every: int = 8, time_limit: float = 3600.0) -> dict[str, Any]:
+                    accuracy_every: int = 8, time_limit: float = 3600.0,
+                    use_graph: bool = True) -> dict[str, Any]:
     """The mix arriving as a Poisson process of ``rate`` requests/s, served by continuous
     batching under an HBM budget; latency = completion - arrival, reported over all solves and
     per outcome, with a sample of results compared with FP64 NumPy solutions."""
@@ -309,7 +312,8 @@ def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
     reqs = make_mix(n, seed, n2=n2, n3=n3, storage=storage, jacobi_cap=jacobi_cap)
     arr = poisson_arrivals(n, rate, seed + 1)
     acc_ids = accuracy_sample(reqs, every=accuracy_every)
-    srv = Server(int(budget_gb * 1e9), round_sweeps=round_sweeps, keep_fields=acc_ids)
+    srv = Server(int(budget_gb * 1e9), round_sweeps=round_sweeps, keep_fields=acc_ids,
+                 use_graph=use_graph)
     try:
         res, st = srv.serve(reqs, arr, time_limit=time_limit)
     finally:

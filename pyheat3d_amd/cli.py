@@ -119,7 +119,7 @@ def _cmd_batch(a: argparse.Namespace) -> int:
 
     r = batch_benchmark(a.n, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
                         a.frac_jacobi, a.storage, a.baseline_every, a.round_sweeps,
-                        a.jacobi_cap, a.accuracy_every)
+                        a.jacobi_cap, a.accuracy_every, use_graph=not a.no_graph)
     print(json.dumps(r, indent=1))
     return 0
 
@@ -128,7 +128,8 @@ def _cmd_serve(a: argparse.Namespace) -> int:
     from .benchmark import serve_benchmark
 
     r = serve_benchmark(a.n, a.rate, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
-                        a.storage, a.budget_gb, a.round_sweeps, a.jacobi_cap, a.accuracy_every)
+                        a.storage, a.budget_gb, a.round_sweeps, a.jacobi_cap, a.accuracy_every,
+                        use_graph=not a.no_graph)
     print(json.dumps(r, indent=1))
     return 0
 
@@ -150,7 +151,15 @@ def _cmd_jacobi(a: argparse.Namespace) -> int:
 
 
 def _cmd_tune(a: argparse.Namespace) -> int:
-    from .tune import tune, update_table
+    from .tune import tune, tune_batch, update_batch_table, update_table
+
+    if a.batch:
+        entries = tune_batch(a.batch_n)
+        for e in entries:
+            print(f"best {e['key']}: " + ", ".join(f"{k}={v}" for k, v in e.items() if k != "key"))
+        if a.write:
+            update_batch_table(entries)
+        return 0
 
     entries = []
     for key in a.variants:
@@ -169,6 +178,9 @@ def _size_args(p: argparse.ArgumentParser) -> None:
     p.add_argument("--n3-min", type=int, default=32)
     p.add_argument("--n3-max", type=int, default=256)
     p.add_argument("--jacobi-cap", type=int, default=10 ** 6, help="sweep cap of Jacobi solves")
+    p.add_argument("--no-graph", action="store_true",
+                   help="enqueue rounds eagerly instead of replaying their HIP graphs (for kernel "
+                        "traces: graph launches of large rounds have crashed under rocprofv3)")
     p.add_argument("--accuracy-every", type=int, default=8,
                    help="compare every k-th result with its FP64 NumPy solution (bounded size)")
     p.add_argument("--storage", default="float32", choices=("float32", "float16", "bfloat16"))
@@ -236,6 +248,9 @@ def main(argv: list[str] | None = None) -> int:
     t.add_argument("--bc", default="periodic", choices=("dirichlet", "periodic"))
     t.add_argument("--source", action="store_true", help="tune with a float32 source term")
     t.add_argument("--write", action="store_true")
+    t.add_argument("--batch", action="store_true",
+                   help="tune the batch's tier-2 passes (tile shapes, chunk policies) instead")
+    t.add_argument("--batch-n", type=int, default=256, help="solves in the tuning mix")
     t.set_defaults(fn=_cmd_tune)
 
     a = p.parse_args(argv)

@@ -28,7 +28,7 @@ constexpr int kPassGroups = 2048; // workgroups of the persistent / looping laun
 constexpr int kSlices = 16;       // residual slices per tier-2 grid
 constexpr int kTier2Tiles = 512; // tier-2 tiles a large grid is spread over
 constexpr int kMaxChunk = 64;     // longest tier-2 march chunk of passes with k < 8
-constexpr int kClassStreams = 3;  // streams for tier-2 classes beside the caller's
+constexpr int kClassStreams = 7;  // streams for tier-2 classes beside the caller's
 constexpr int kR3 = 4, kW3 = 4;   // tier-2 3D tile: 4 waves x 4 rows, 64 sites wide
 constexpr int kR2 = 4, kW2 = 4;   // tier-2 2D tile: 4 waves x 64 columns, 4 rows per step
 constexpr int kTbR3 = 8, kTbW3 = 4; // tier-2 3D temporal-blocking tile: 4 waves x 8 rows
@@ -293,18 +293,52 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 
 // ---- tier 2 ----------------------------------------------------------------------------------
 
-// Tier-2 tiling of one grid: z-march / y-march tiles for single steps, overlapped
-// temporal-blocking tiles (3D: 64 x 32 extended, 2D: 4 strips of 64 columns) for K > 1. The
-// march chunk is chosen per grid so that a grid spreads over about kTier2Tiles tiles (a lone
-// large grid would otherwise run a few long, load-latency-bound marches), but no shorter than
-// max(8, 2K) planes / rows. Results do not depend on the tiling.
+// Tier-2 tile shapes. Single steps: the z-march tile (3D, 4 waves x 4 rows, 64 sites wide) or
+// the y-march tile (2D, 4 strips of 64 columns). Fused steps (float32, K > 1), by shape id
+// (ph3_solve_desc.tile; 0 = the library's rule, see default_tile):
+//   3D  1: 1-site lanes, 4 waves x 8 rows (64 x 32 extended tile, 256 lanes)
+//       2: 2-site lanes, 4 waves x 8 rows (128 x 32)
+//       3: 1-site lanes, 8 waves x 8 rows (64 x 64, 512 lanes: the single-grid tb4 shape)
+//       4: 2-site lanes, 8 waves x 4 rows (128 x 32, 512 lanes)
+//   2D  1: 1-site lanes, masked stores        2: 2-site lanes, masked stores
+//       3: 1-site lanes, a store every step   4: 2-site lanes, a store every step
+//       (4 strips per tile; store modes as in tb_tiles.h StoreMode)
+// A wider shape recomputes a smaller share of halo sites; the narrow ones split small grids
+// into more tiles. Results never depend on the shape.
+struct T2Shape {
+    int v, r, w, sm;   // sites per lane, rows per wave (3D), waves per workgroup, store mode
+};
+
+__host__ __device__ inline T2Shape tier2_shape(int dims, int K, int id) {
+    if (K <= 1) return T2Shape{1, dims == 3 ? kR3 : kR2, dims == 3 ? kW3 : kW2, kStoreMasked};
+    if (dims == 3) {
+        switch (id) {
+            case 2: return T2Shape{2, 8, 4, kStoreMasked};
+            case 3: return T2Shape{1, 8, 8, kStoreMasked};
+            case 4: return T2Shape{2, 4, 8, kStoreMasked};
+            default: return T2Shape{1, 8, 4, kStoreMasked};
+        }
+    }
+    switch (id) {
+        case 2: return T2Shape{2, 0, kW2, kStoreMasked};
+        case 3: return T2Shape{1, 0, kW2, kStoreAll};
+        case 4: return T2Shape{2, 0, kW2, kStoreAll};
+        default: return T2Shape{1, 0, kW2, kStoreMasked};
+    }
+}
+
+// Tier-2 tiling of one grid. The march chunk is chosen so that the grid spreads over about
+// `tpg` tiles, a per-class figure the host derives from how many grids share the class (a lone
+// large grid needs many short marches to fill the device; many grids together fill it with
+// long ones, which recompute fewer of the 2K pipeline-fill planes of a fused pass), within
+// [lo, hi] planes or rows. Results do not depend on the tiling.
 struct T2Geom {
     int tx, ty, chunk;  // tiles along x, along y (1 in 2D), march chunk
     long long tiles;
 };
 
 __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K, int few,
-                                             int V) {
+                                             const T2Shape& sh, int tpg, int max_chunk) {
     T2Geom g;
     if (dims == 3) {
         if (K <= 1) {
@@ -312,27 +346,26 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
             g.ty = (ny + kR3 * kW3 - 1) / (kR3 * kW3);
         } else {
             // a tile of 64*V columns writes 64*V - 2*KV of them (TbGeom)
-            const int kv = ((K + V - 1) / V) * V, ox = 64 * V - 2 * kv;
+            const int kv = ((K + sh.v - 1) / sh.v) * sh.v, ox = 64 * sh.v - 2 * kv;
+            const int oy = sh.w * sh.r - 2 * K;
             g.tx = (nx + ox - 1) / ox;
-            g.ty = (ny + kTbR3 * kTbW3 - 2 * K - 1) / (kTbR3 * kTbW3 - 2 * K);
+            g.ty = (ny + oy - 1) / oy;
         }
     } else {
-        const int ox = K <= 1 ? 64 : 64 - 2 * K;
-        g.tx = ((nx + ox - 1) / ox + kW2 - 1) / kW2;
+        const int kv = ((K + sh.v - 1) / sh.v) * sh.v;
+        const int ox = K <= 1 ? 64 : 64 * sh.v - 2 * kv;
+        g.tx = ((nx + ox - 1) / ox + sh.w - 1) / sh.w;
         g.ty = 1;
     }
     const int march = dims == 3 ? nz : ny;
     const long long txy = (long long)g.tx * g.ty;
-    const long long want = (kTier2Tiles + txy - 1) / txy;
+    const long long want = (tpg + txy - 1) / txy;
     int ch = (int)((march + want - 1) / want);
-    // A k-step chunk recomputes 2k extra rows or planes, so deep 2D passes (k = 8) take chunks
-    // of 64-128 rows (measured on mixes of 2048 2D/3D solves: 7.6% faster with half of them
-    // Jacobi, 2.6% with all heat, than 16-64); shallower passes keep 8-64, where longer chunks
-    // measured slower on large 3D heat grids. `few`: the class has fewer tiles than a pass has
-    // workgroups, so a pass lasts as long as one tile's march and deep 2D passes take chunks
-    // from 2k rows (more, shorter tiles) instead.
+    // A k-step chunk recomputes 2k extra rows or planes. Deep 2D passes (k = 8) take chunks of
+    // 64-128 rows; `few` (the class has fewer tiles than a pass has workgroups: a pass lasts as
+    // long as one tile's march) lets them go down to 2k rows instead.
     const int lo = K >= 8 ? (few ? 2 * K : 8 * K) : (K <= 1 ? 8 : (2 * K > 8 ? 2 * K : 8));
-    const int hi = K >= 8 ? 16 * K : kMaxChunk;
+    const int hi = max_chunk > 0 ? (max_chunk > lo ? max_chunk : lo) : (K >= 8 ? 16 * K : kMaxChunk);
     if (ch < lo) ch = lo;
     if (ch > hi) ch = hi;
     g.chunk = ch;
@@ -340,13 +373,14 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
     return g;
 }
 
-// V: sites per lane; 2 only for the class of float32 3D 2-step grids of even width (see
-// ph3_batch_set), 1 otherwise.
-template <typename T, bool P, bool S, int DIMS, int K, int V>
-__global__ void __launch_bounds__(kWG)
+// One pass of a tier-2 class: V sites per lane, R rows per wave (3D fused), W waves, store mode
+// SM, as tier2_shape gives them for the class's shape id.
+template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM>
+__global__ void __launch_bounds__(64 * W)
 batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
-           const long long* __restrict__ prefix, int n, int p, int few) {
-    static_assert(V == 1 || (DIMS == 3 && K == 2), "2-site lanes only for 3D 2-step passes");
+           const long long* __restrict__ prefix, int n, int p, int few, int tpg,
+           int max_chunk) {
+    const T2Shape sh{V, R, W, SM};
     const long long total = prefix[n];
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = n - 1;  // last i with prefix[i] <= t
@@ -360,24 +394,24 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
             const int cur = (int)((d.done / K + p) & 1);
             const T* u = (const T*)d.buf[cur];
             T* o = (T*)d.buf[cur ^ 1];
-            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K, few, V);
+            const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K, few, sh, tpg, max_chunk);
             const int ix = (int)(local % g.tx);
             const long long rest = local / g.tx;
-            if (DIMS == 3) {
+            if constexpr (DIMS == 3) {
                 const int iy = (int)(rest % g.ty), iz = (int)(rest / g.ty);
                 if constexpr (K == 1)
                     zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
                                                         d.c1, g.chunk, ix, iy, iz, 0);
                 else
-                    tb3d_tile<T, P, S, V, kTbR3, kTbW3, K>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
-                                                           d.c1, g.chunk, ix, iy, iz);
+                    tb3d_tile<T, P, S, V, R, W, K, SM>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
+                                                      d.c1, g.chunk, ix, iy, iz);
             } else {
                 if constexpr (K == 1)
                     ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1,
                                                         g.chunk, ix, (int)rest);
                 else
-                    tb2d_tile<T, P, S, 1, K, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1, g.chunk,
-                                                  ix, (int)rest);
+                    tb2d_tile<T, P, S, V, K, W, SM>(u, o, d.src, d.ny, d.nx, d.c0, d.c1,
+                                                   g.chunk, ix, (int)rest);
             }
         }
         __syncthreads();  // the 3D tiles' LDS slot is reused by the next tile
@@ -435,8 +469,10 @@ __global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list,
 
 struct Tier2Class {
     int storage, dims, periodic, src, kpass;
-    int vec = 1;   // sites per lane of the class's passes
+    int shape = 1; // tier2_shape id of the class's fused passes (1 for single steps)
     int few = 0;   // tier2_geom's short-chunk mode (set from the class's tile count)
+    int tpg = ph3::kTier2Tiles;  // tiles per grid the chunk aims at
+    int max_chunk = 0;           // longest march chunk (0: the geometry's default)
     std::vector<int> idx;
     const int* d_list = nullptr;          // into the batch's int arena
     const long long* d_prefix = nullptr;  // into the batch's prefix arena
@@ -468,6 +504,10 @@ struct ph3_batch {
     // tier-2 classes after the first run their passes on these, beside each other
     hipStream_t cls[ph3::kClassStreams] = {};
     hipEvent_t seg = nullptr, cls_join[ph3::kClassStreams] = {};
+    // chunk policy per (dims - 2, log2 kpass): tiles a pass of the class aims at, and the
+    // longest chunk (0 = defaults); set by ph3_batch_params
+    int target_tiles[2][4] = {};
+    int max_chunk[2][4] = {};
 };
 
 // Forget the current set (the batch then has no solves; rounds enqueue nothing).
@@ -538,6 +578,7 @@ static int check_desc(const ph3_solve_desc& d, int i) {
     if (d.storage < 0 || d.storage > 2) { set_error("solve %d: bad storage", i); return PH3_EINVAL; }
     if (!d.buf[0] || !d.buf[1] || d.buf[0] == d.buf[1]) { set_error("solve %d: buffers", i); return PH3_EINVAL; }
     if (d.mode != 0 && d.mode != 1) { set_error("solve %d: mode %d", i, d.mode); return PH3_EINVAL; }
+    if (d.tile < 0 || d.tile > 4) { set_error("solve %d: tile %d not in 0..4", i, d.tile); return PH3_EINVAL; }
     if (d.kpass < 0 || d.kpass > 8 || (d.kpass & (d.kpass - 1))) {
         set_error("solve %d: kpass %d not in {0, 1, 2, 4, 8}", i, d.kpass);
         return PH3_EINVAL;
@@ -566,6 +607,35 @@ static int choose_kpass(const ph3_solve_desc& d) {
     return k;
 }
 
+// Tile shape of a tier-2 solve (tier2_shape ids). The caller's `tile` is taken when it is one of
+// the fused shapes; 2-site lanes need an even width above 64 and 8-byte aligned buffers, and
+// fall back to the 1-site shape of the same width class otherwise. Default (tile 0): 2-site
+// lanes for float32 3D 2-step grids where allowed (a 128-wide tile then idles no more lanes
+// than two 64-wide ones), the narrow 1-site shape otherwise.
+static int tier2_shape_for(const ph3_solve_desc& d) {
+    if (d.kpass <= 1) return 1;
+    auto al8 = [](const void* q) { return ((uintptr_t)q & 7u) == 0; };
+    const bool two = d.nx > 64 && d.nx % 2 == 0 && al8(d.buf[0]) && al8(d.buf[1]) &&
+                     (!d.src || al8(d.src));
+    int id = d.tile;
+    if (id == 0) id = d.dims == 3 && d.kpass == 2 ? 2 : 1;
+    if (!two && (id == 2 || id == 4)) id -= 1;
+    return id;
+}
+
+extern "C" int ph3_batch_params(ph3_batch* b, int dims, int kpass, int target_tiles, int max_chunk) {
+    if (!b || (dims != 2 && dims != 3) || kpass < 1 || kpass > 8 || (kpass & (kpass - 1)) ||
+        target_tiles < 0 || max_chunk < 0 || max_chunk > 4096) {
+        set_error("ph3_batch_params: bad arguments");
+        return PH3_EINVAL;
+    }
+    int lk = 0;
+    while ((1 << lk) < kpass) ++lk;
+    b->target_tiles[dims - 2][lk] = target_tiles;
+    b->max_chunk[dims - 2][lk] = max_chunk;
+    return PH3_OK;
+}
+
 extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, void* stream) {
     if (!b || n < 0 || n > b->cap || (n && !descs)) { set_error("ph3_batch_set: bad arguments"); return PH3_EINVAL; }
     hipStream_t s = (hipStream_t)stream;
@@ -585,19 +655,16 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         const long long sites = (long long)d.nz * d.ny * d.nx;
         if (sites <= PH3_BATCH_SMALL_SITES) { small[d.storage].push_back(i); continue; }
         large.push_back(i);
-        // 2-site lanes for float32 3D 2-step grids of even width above 64 (a 128-wide tile
-        // then idles no more lanes than two 64-wide ones) with 8-byte aligned buffers
-        auto al8 = [](const void* q) { return ((uintptr_t)q & 7u) == 0; };
-        const int vec = d.dims == 3 && d.kpass == 2 && d.storage == ph3::kF32 && d.nx > 64 &&
-                        d.nx % 2 == 0 && al8(d.buf[0]) && al8(d.buf[1]) &&
-                        (!d.src || al8(d.src)) ? 2 : 1;
+        const int shape = tier2_shape_for(d);
         Tier2Class* c = nullptr;
         for (auto& k : classes)
             if (k.storage == d.storage && k.dims == d.dims && k.periodic == (d.periodic != 0) &&
-                k.src == (d.src != nullptr) && k.kpass == d.kpass && k.vec == vec) c = &k;
+                k.src == (d.src != nullptr) && k.kpass == d.kpass && k.shape == shape) c = &k;
         if (!c) {
-            classes.push_back(Tier2Class{d.storage, d.dims, d.periodic != 0, d.src != nullptr,
-                                         d.kpass, vec, 0, {}});
+            Tier2Class k;
+            k.storage = d.storage; k.dims = d.dims; k.periodic = d.periodic != 0;
+            k.src = d.src != nullptr; k.kpass = d.kpass; k.shape = shape;
+            classes.push_back(k);
             c = &classes.back();
         }
         c->idx.push_back(i);
@@ -612,10 +679,20 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     std::vector<size_t> o_list(classes.size()), o_prefix(classes.size());
     for (size_t ci = 0; ci < classes.size(); ++ci) {
         Tier2Class& c = classes[ci];
+        const ph3::T2Shape sh = ph3::tier2_shape(c.dims, c.kpass, c.shape);
+        int lk = 0;
+        while ((1 << lk) < c.kpass) ++lk;
+        const int target = b->target_tiles[c.dims - 2][lk] > 0 ? b->target_tiles[c.dims - 2][lk]
+                                                                : ph3::kTier2Tiles;
+        // tiles per grid: the class's target split over its grids, at most the lone-grid figure
+        const long long per = (target + (long long)c.idx.size() - 1) / (long long)c.idx.size();
+        c.tpg = (int)(per > ph3::kTier2Tiles ? ph3::kTier2Tiles : (per < 1 ? 1 : per));
+        if (b->target_tiles[c.dims - 2][lk] <= 0) c.tpg = ph3::kTier2Tiles;   // default policy
+        c.max_chunk = b->max_chunk[c.dims - 2][lk];
         long long total = 0;
         for (int i : c.idx) {
             const ph3_solve_desc& d = table[i];
-            total += ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, 0, c.vec).tiles;
+            total += ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, 0, sh, c.tpg, c.max_chunk).tiles;
         }
         c.few = total < ph3::kPassGroups ? 1 : 0;
         o_list[ci] = put(c.idx);
@@ -623,7 +700,8 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         lh.push_back(0);
         for (int i : c.idx) {
             const ph3_solve_desc& d = table[i];
-            lh.push_back(lh.back() + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, c.vec).tiles);
+            lh.push_back(lh.back() + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, sh,
+                                                     c.tpg, c.max_chunk).tiles);
         }
     }
     if (ih.size() > 4 * (size_t)b->cap || lh.size() > (size_t)b->cap + kMaxClasses) {
@@ -675,30 +753,44 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     return PH3_OK;
 }
 
-template <typename T, bool P, bool S, int DIMS, int K, int V = 1>
+template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM>
 static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
-    ph3::batch_pass<T, P, S, DIMS, K, V><<<(unsigned)ph3::kPassGroups, ph3::kWG, 0, s>>>(
-        b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p, c.few);
+    ph3::batch_pass<T, P, S, DIMS, K, V, R, W, SM><<<(unsigned)ph3::kPassGroups, 64 * W, 0, s>>>(
+        b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p, c.few, c.tpg, c.max_chunk);
+}
+
+// Fused passes of one K by shape id (tier2_shape).
+template <typename T, bool P, bool S, int DIMS, int K>
+static void pass_shape(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
+    using namespace ph3;
+    if constexpr (DIMS == 3) {
+        switch (c.shape) {
+            case 2: return launch_pass<T, P, S, 3, K, 2, 8, 4, kStoreMasked>(c, b, p, s);
+            case 3: return launch_pass<T, P, S, 3, K, 1, 8, 8, kStoreMasked>(c, b, p, s);
+            case 4: return launch_pass<T, P, S, 3, K, 2, 4, 8, kStoreMasked>(c, b, p, s);
+            default: return launch_pass<T, P, S, 3, K, 1, 8, 4, kStoreMasked>(c, b, p, s);
+        }
+    } else {
+        switch (c.shape) {
+            case 2: return launch_pass<T, P, S, 2, K, 2, 0, kW2, kStoreMasked>(c, b, p, s);
+            case 3: return launch_pass<T, P, S, 2, K, 1, 0, kW2, kStoreAll>(c, b, p, s);
+            case 4: return launch_pass<T, P, S, 2, K, 2, 0, kW2, kStoreAll>(c, b, p, s);
+            default: return launch_pass<T, P, S, 2, K, 1, 0, kW2, kStoreMasked>(c, b, p, s);
+        }
+    }
 }
 
 template <typename T, bool P, bool S, int DIMS>
 static void pass_k(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
+    using namespace ph3;
     if constexpr (std::is_same<T, float>::value) {
-        if (c.kpass == 2) {
-            if constexpr (DIMS == 3) {
-                if (c.vec == 2) return launch_pass<T, P, S, DIMS, 2, 2>(c, b, p, s);
-            }
-            return launch_pass<T, P, S, DIMS, 2>(c, b, p, s);
-        }
-        if constexpr (DIMS == 3) {
-            if (c.kpass == 4) return launch_pass<T, P, S, DIMS, 4>(c, b, p, s);
-        }
+        if (c.kpass == 2) return pass_shape<T, P, S, DIMS, 2>(c, b, p, s);
+        if (c.kpass == 4) return pass_shape<T, P, S, DIMS, 4>(c, b, p, s);
         if constexpr (DIMS == 2) {
-            if (c.kpass == 4) return launch_pass<T, P, S, DIMS, 4>(c, b, p, s);
-            if (c.kpass == 8) return launch_pass<T, P, S, DIMS, 8>(c, b, p, s);
+            if (c.kpass == 8) return pass_shape<T, P, S, DIMS, 8>(c, b, p, s);
         }
     }
-    launch_pass<T, P, S, DIMS, 1>(c, b, p, s);
+    launch_pass<T, P, S, DIMS, 1, 1, DIMS == 3 ? kR3 : kR2, DIMS == 3 ? kW3 : kW2, kStoreMasked>(c, b, p, s);
 }
 
 template <typename T>

@@ -26,7 +26,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _hip
-from .batch import STATUS, Batch, DeviceFields, Request, Result, current_buffer, make_desc
+from .batch import (STATUS, Batch, DeviceFields, Request, Result, apply_tuned_tiles,
+                    current_buffer, make_desc)
 
 
 def poisson_arrivals(n: int, rate: float, seed: int = 7) -> np.ndarray:
@@ -63,6 +64,10 @@ class Server:
         self.keep_fields = keep_fields
         self.stream = _hip.Stream()
         self.batch = Batch(capacity, self.stream)
+        from .batch import default_params
+
+        for pr in default_params():
+            self.batch.params(*pr)
         self.capacity = capacity
 
     def close(self) -> None:
@@ -72,6 +77,7 @@ class Server:
     def serve(self, requests: list[Request], arrivals: np.ndarray,
               time_limit: float = 600.0) -> tuple[list[Result], ServeStats]:
         """Serve ``requests`` arriving at ``arrivals`` (seconds after start); blocking."""
+        apply_tuned_tiles(requests)
         order = np.argsort(arrivals, kind="stable")
         pending = [(float(arrivals[i]), requests[i]) for i in order]
         for _, r in pending:

@@ -145,3 +145,149 @@ def update_table(entries: list[dict[str, Any]], path: Path = TABLE) -> None:
     path.parent.mkdir(parents=True, exist_ok=True)
     path.write_text(json.dumps(table, indent=1, sort_keys=True))
     _table = None
+
+
+# ---- batched solves -------------------------------------------------------------------------
+# Two more kinds of entries tune the batch's tier-2 passes (ph3_batch_*), which the reference
+# does not have (it times one grid at a time):
+#   batch/tile/<d>d/b<bucket>     {"tile": id}: the tile shape of a solve's fused passes
+#                                 (ph3_solve_desc.tile), a function of the solve alone;
+#   batch/policy/<d>d/k<k>        {"target_tiles": T, "max_chunk": C}: how the (dims, kpass)
+#                                 classes cut their grids into tiles (ph3_batch_params).
+# Neither changes a result: tile shape and chunk never change a site's arithmetic.
+
+def batch_tile(dims: int, shape: tuple[int, ...]) -> int:
+    """Tuned tier-2 tile shape id of a solve (0: the library's rule)."""
+    e = load_table().get(f"batch/tile/{dims}d/b{bucket(shape)}")
+    return int(e["tile"]) if e else 0
+
+
+def batch_policy() -> list[tuple[int, int, int, int]]:
+    """(dims, kpass, target_tiles, max_chunk) of every tuned batch class policy."""
+    out = []
+    for key, e in load_table().items():
+        if key.startswith("batch/policy/"):
+            _, _, d, k = key.split("/")
+            out.append((int(d[:-1]), int(k[1:]), int(e["target_tiles"]), int(e["max_chunk"])))
+    return sorted(out)
+
+
+#: the fused classes of the float32 batch (dims, kpass) and the grid sizes that land in them
+#: (2D: 8-step passes from 4097 sites; 3D: 2-step passes below 128^3 sites, 4-step from there)
+BATCH_CLASSES = {(2, 8): (65, 2048), (3, 2): (17, 127), (3, 4): (128, 256)}
+#: (target_tiles, max_chunk) candidates per dims; (0, 0) is the library's default
+BATCH_POLICIES = {2: ((0, 0), (4096, 256), (8192, 256), (16384, 256), (4096, 128), (8192, 512)),
+                  3: ((0, 0), (1024, 128), (2048, 128), (4096, 128), (8192, 128), (4096, 256),
+                      (8192, 256))}
+BATCH_SHAPES = (1, 2, 3, 4)
+
+
+def tuning_mix(n: int = 256, seed: int = 77, jacobi_cap: int = 2048, small: bool = False):
+    """A mix the batch is tuned on: the mixed workload's size ranges (60% 2D N in 64..2048, 40%
+    3D N in 32..256), half heat, half Jacobi with a short sweep cap; ``small``: the same with
+    2D N up to 256 and 3D N up to 64 (many small grids)."""
+    from .batch import make_mix
+
+    return make_mix(n, seed, n2=(64, 256 if small else 2048), n3=(32, 64 if small else 256),
+                    frac_jacobi=0.5, jacobi_cap=jacobi_cap)
+
+
+def _class_of(req) -> tuple[int, int] | None:
+    """The fused class a float32 request of the tuning mixes lands in (None: tier 1)."""
+    if req.sites <= _hip.BATCH_SMALL_SITES:
+        return None
+    if req.dims == 2:
+        return (2, 8)
+    return (3, 4) if req.sites >= (1 << 21) else (3, 2)
+
+
+def _half(req, c) -> int:
+    """0 for the lower half of a class's size buckets, 1 for the upper half."""
+    lo, hi = BATCH_CLASSES[c]
+    b0, b1 = bucket((lo,) * c[0]), bucket((hi,) * c[0])
+    return 1 if bucket(req.shape) > (b0 + b1) // 2 else 0
+
+
+def _time_mixes(policy: dict, shapes: dict, n: int, reps: int = 2) -> tuple[float, float]:
+    from .batch import BatchSolver
+
+    out = []
+    for small in (False, True):
+        best = None
+        for _ in range(reps):
+            reqs = tuning_mix(n, small=small)
+            for r in reqs:
+                c = _class_of(r)
+                r.tile = shapes.get((c, _half(r, c)), 0) if c else 0   # 0: the library's rule
+            params = [(d, k, t, ch) for (d, k), (t, ch) in policy.items()]
+            b = BatchSolver(reqs, round_sweeps=256, params=params, tuned_tiles=False)
+            try:
+                sec = b.run()["seconds"]
+            finally:
+                b.close()
+            best = sec if best is None else min(best, sec)
+        out.append(best)
+    return out[0], out[1]
+
+
+def tune_batch(n: int = 256, verbose: bool = True) -> list[dict[str, Any]]:
+    """Autotune the batch's tier-2 passes on two mixes (:func:`tuning_mix`, full size ranges
+    and small grids): coordinate descent over each fused class's chunk policy (target tiles per
+    class, longest chunk) and the tile shape of the lower and upper half of its size range,
+    timing whole batch runs (best of 2). The score is the sum of both mixes' times relative to
+    the library's defaults. Returns table entries for ``update_batch_table``."""
+    policy = {c: (0, 0) for c in BATCH_CLASSES}
+    shapes: dict = {(c, h): 0 for c in BATCH_CLASSES for h in (0, 1)}
+    base = _time_mixes(policy, shapes, n)
+    score = lambda t: t[0] / base[0] + t[1] / base[1]   # noqa: E731
+    best_t, best = base, 2.0
+    if verbose:
+        print(f"  batch tuning mixes ({n} solves), library defaults: {base[0]:.3f} s (full), "
+              f"{base[1]:.3f} s (small)", flush=True)
+    for sweep in range(2):
+        for c in BATCH_CLASSES:
+            for cand in BATCH_POLICIES[c[0]]:
+                if cand == policy[c]:
+                    continue
+                trial = {**policy, c: cand}
+                t = _time_mixes(trial, shapes, n)
+                if verbose:
+                    print(f"  sweep {sweep} class {c} policy {cand}: {t[0]:.3f} / {t[1]:.3f} s", flush=True)
+                if score(t) < best:
+                    best, best_t, policy = score(t), t, trial
+            for h in (0, 1):
+                for sid in BATCH_SHAPES:
+                    if sid == shapes[(c, h)]:
+                        continue
+                    trial = {**shapes, (c, h): sid}
+                    t = _time_mixes(policy, trial, n)
+                    if verbose:
+                        print(f"  sweep {sweep} class {c} half {h} shape {sid}: {t[0]:.3f} / {t[1]:.3f} s", flush=True)
+                    if score(t) < best:
+                        best, best_t, shapes = score(t), t, trial
+    if verbose:
+        print(f"  tuned: {best_t[0]:.3f} s (full), {best_t[1]:.3f} s (small)", flush=True)
+    entries = []
+    for c, (lo, hi) in BATCH_CLASSES.items():
+        d, k = c
+        t, ch = policy[c]
+        entries.append({"key": f"batch/policy/{d}d/k{k}", "target_tiles": t, "max_chunk": ch,
+                        "mix_seconds": [round(x, 4) for x in best_t],
+                        "default_mix_seconds": [round(x, 4) for x in base], "mix_solves": n})
+        b0, b1 = bucket((lo,) * d), bucket((hi,) * d)
+        for bk in range(b0, b1 + 1):
+            h = 1 if bk > (b0 + b1) // 2 else 0
+            entries.append({"key": f"batch/tile/{d}d/b{bk}", "tile": shapes[(c, h)]})
+    return entries
+
+
+def update_batch_table(entries: list[dict[str, Any]], path: Path = TABLE) -> None:
+    """Write batch entries (``tune_batch``) into the tuning table."""
+    global _table
+    table = json.loads(path.read_text()) if path.exists() else {}
+    arch = _hip.device_info(0)["arch"]
+    for e in entries:
+        table[e["key"]] = {**{k: v for k, v in e.items() if k != "key"}, "arch": arch}
+    path.parent.mkdir(parents=True, exist_ok=True)
+    path.write_text(json.dumps(table, indent=1, sort_keys=True))
+    _table = None

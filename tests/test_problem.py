@@ -183,7 +183,7 @@ def test_tuned_tile_needs_a_lane_vector_that_divides_the_row():
 
     checked = 0
     for key, e in tune.load_table().items():
-        if e["tile"]["vec"] <= 1:
+        if key.startswith("batch/") or e["tile"]["vec"] <= 1:   # batch entries: tile ids
             continue
         kind, kk, dims, storage, bc = key.split("/")[:5]
         shape = tuple(e["shape"])
@@ -208,3 +208,20 @@ def test_bench_cpu_solve_seconds_runs_the_oracle():
         st = q.stencil()
         t = bench._cpu_solve_seconds((q.shape, q.seed, q.kind, st.c0, st.c1, q.mode, sweeps))
         assert 0.0 < t < 5.0
+
+
+def test_batch_table_entries_are_valid():
+    """Batch entries of the tuning table: tile shape ids 0-4 for every size bucket of each fused
+    class, and chunk policies the library accepts (read back as ph3_batch_params arguments)."""
+    from pyheat3d_amd import tune
+
+    table = tune.load_table()
+    tiles = {k: e for k, e in table.items() if k.startswith("batch/tile/")}
+    assert tiles and all(0 <= e["tile"] <= 4 for e in tiles.values())
+    for (d, _k), (lo, hi) in tune.BATCH_CLASSES.items():
+        for b in range(tune.bucket((lo,) * d), tune.bucket((hi,) * d) + 1):
+            assert f"batch/tile/{d}d/b{b}" in tiles
+    pol = tune.batch_policy()
+    assert {(d, k) for d, k, _, _ in pol} == set(tune.BATCH_CLASSES)
+    assert all(t >= 0 and 0 <= c <= 4096 for _, _, t, c in pol)
+    assert tune.batch_tile(3, (256, 256, 256)) == table["batch/tile/3d/b24"]["tile"]
