@@ -183,7 +183,8 @@ typedef struct ph3_solve_desc {
     int tile;                    /* tier-2 tile shape of fused passes: 0 = the library's rule;
                                     3D 1: 1-site lanes, 32 rows; 2: 2-site lanes, 32 rows;
                                     3: 1-site lanes, 64 rows; 4: 2-site lanes, 32 rows in 8
-                                    waves. 2D 1/2: 1/2-site lanes; 3/4: the same with a store
+                                    waves; 5/6: shapes 1/3 with buffer stores on the stored
+                                    planes only. 2D 1/2: 1/2-site lanes; 3/4: the same with a store
                                     on every step. 2-site shapes need an even width above 64
                                     and 8-byte aligned buffers (else the 1-site shape). Never
                                     changes the result. */

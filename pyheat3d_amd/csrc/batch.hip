@@ -25,7 +25,9 @@ namespace ph3 {
 
 constexpr int kWG = 256;          // lanes per workgroup in every batch kernel
 constexpr int kPassGroups = 2048; // workgroups of the persistent / looping launches
-constexpr int kSlices = 16;       // residual slices per tier-2 grid
+constexpr int kMinSlices = 16;    // residual slices of a tier-2 Jacobi grid: at least ...
+constexpr int kMaxSlices = 256;   // ... and at most these, one per kSliceSites sites
+constexpr int kSliceSites = 32768;
 constexpr int kTier2Tiles = 512; // tier-2 tiles a large grid is spread over
 constexpr int kMaxChunk = 64;     // longest tier-2 march chunk of passes with k < 8
 constexpr int kClassStreams = 7;  // streams for tier-2 classes beside the caller's
@@ -300,6 +302,7 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 //       2: 2-site lanes, 4 waves x 8 rows (128 x 32)
 //       3: 1-site lanes, 8 waves x 8 rows (64 x 64, 512 lanes: the single-grid tb4 shape)
 //       4: 2-site lanes, 8 waves x 4 rows (128 x 32, 512 lanes)
+//       5, 6: shapes 1 and 3 storing through buffers on the stored planes only (StoreMode)
 //   2D  1: 1-site lanes, masked stores        2: 2-site lanes, masked stores
 //       3: 1-site lanes, a store every step   4: 2-site lanes, a store every step
 //       (4 strips per tile; store modes as in tb_tiles.h StoreMode)
@@ -316,6 +319,8 @@ __host__ __device__ inline T2Shape tier2_shape(int dims, int K, int id) {
             case 2: return T2Shape{2, 8, 4, kStoreMasked};
             case 3: return T2Shape{1, 8, 8, kStoreMasked};
             case 4: return T2Shape{2, 4, 8, kStoreMasked};
+            case 5: return T2Shape{1, 8, 4, kStorePlanes};
+            case 6: return T2Shape{1, 8, 8, kStorePlanes};
             default: return T2Shape{1, 8, 4, kStoreMasked};
         }
     }
@@ -382,6 +387,9 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
            int max_chunk) {
     const T2Shape sh{V, R, W, SM};
     const long long total = prefix[n];
+    // Tiles are dealt round-robin by workgroup index. (Dealing each XCD a contiguous block of
+    // every window of gridDim.x tiles, so neighbours share an L2, measured slower: full-range
+    // bench mix 100 -> 79 solves/s.)
     for (long long t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = n - 1;  // last i with prefix[i] <= t
         while (lo < hi) {
@@ -425,13 +433,29 @@ __device__ inline long long done_after_segment(const ph3_solve_desc& d) {
     return d.done + adv;
 }
 
+// Residual slices of a tier-2 solve: 0 for heat; for Jacobi one per kSliceSites sites, within
+// [kMinSlices, kMaxSlices]. A function of the solve alone, so its FP64 summation order (slices
+// in order, each a fixed workgroup tree over interleaved rows) is the same in any batch.
+__host__ __device__ inline int res_slices(const ph3_solve_desc& d) {
+    if (d.mode != 1) return 0;
+    const long long n = ((long long)d.nz * d.ny * d.nx + kSliceSites - 1) / kSliceSites;
+    return (int)(n < kMinSlices ? kMinSlices : (n > kMaxSlices ? kMaxSlices : n));
+}
+
+// Slice `item` of the listed solves: `prefix` holds each solve's first slice (res_slices).
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
-               double* __restrict__ partial) {
+batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list,
+               const int* __restrict__ prefix, int n, double* __restrict__ partial) {
     __shared__ double sm[kWG];
-    for (int item = blockIdx.x; item < n * kSlices; item += gridDim.x) {
-        const int t = item / kSlices, sl = item - t * kSlices;
+    const int total = prefix[n];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = n - 1;  // last t with prefix[t] <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= item) lo = mid; else hi = mid - 1;
+        }
+        const int t = lo, sl = item - prefix[t], nsl = prefix[t + 1] - prefix[t];
         const ph3_solve_desc& d = descs[list[t]];
         if (d.mode != 1 || d.status != 0 || d.storage != Codec<T>::code) continue;
         // the state the segment's passes have written (the counters advance in batch_check)
@@ -439,7 +463,7 @@ batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
         const double part = residual_rows<T>((const T*)d.buf[cur], d.src, d.dims, d.nz,
                                              d.ny, d.nx, d.periodic != 0, d.c0, d.c1,
                                              sl * (kWG / 64) + (threadIdx.x >> 6),
-                                             kSlices * (kWG / 64), threadIdx.x & 63, 64);
+                                             nsl * (kWG / 64), threadIdx.x & 63, 64);
         const double s = wg_sum(part, sm);
         if (threadIdx.x == 0) partial[item] = s;
     }
@@ -447,7 +471,8 @@ batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list, int n,
 
 // After a segment: advance the step counter of every running solve, finish heat solves that
 // reached their step count, and apply the convergence / stagnation / cap test to Jacobi ones.
-__global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list, int n,
+__global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list,
+                            const int* __restrict__ prefix, int n,
                             const double* __restrict__ partial) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -459,7 +484,7 @@ __global__ void batch_check(ph3_solve_desc* descs, const int* __restrict__ list,
         return;
     }
     double r = 0.0;
-    for (int s = 0; s < kSlices; ++s) r += partial[t * kSlices + s];
+    for (int s = prefix[t]; s < prefix[t + 1]; ++s) r += partial[s];
     apply_check(d, r);
 }
 
@@ -473,6 +498,7 @@ struct Tier2Class {
     int few = 0;   // tier2_geom's short-chunk mode (set from the class's tile count)
     int tpg = ph3::kTier2Tiles;  // tiles per grid the chunk aims at
     int max_chunk = 0;           // longest march chunk (0: the geometry's default)
+    long long tiles = 0;         // tiles of one pass over every grid of the class
     std::vector<int> idx;
     const int* d_list = nullptr;          // into the batch's int arena
     const long long* d_prefix = nullptr;  // into the batch's prefix arena
@@ -488,9 +514,10 @@ struct ph3_batch {
     int* d_counter = nullptr;
     double* d_partial = nullptr;
     // Per-set index tables live in two arenas sized for `cap` solves at creation: every list
-    // of a set (tier-1 lists, initial-residual lists, the tier-2 list, the per-class lists)
-    // together holds at most 4 * cap ints, and the per-class tile prefixes at most
-    // cap + kMaxClasses long longs. A set rewrites them in place and never frees device
+    // of a set (tier-1 lists, initial-residual lists, the tier-2 list, the per-class lists, the
+    // residual slice prefix) together holds at most 5 * cap + 1 ints, and the per-class tile
+    // prefixes at most cap + kMaxClasses long longs; the residual partials kMaxSlices * cap
+    // doubles. A set rewrites them in place and never frees device
     // memory, so a graph recorded from an earlier round never points at freed memory.
     int* d_iarena = nullptr;
     long long* d_llarena = nullptr;
@@ -498,6 +525,8 @@ struct ph3_batch {
     int n_small[3] = {0, 0, 0};
     const int* d_large = nullptr;                         // every tier-2 solve
     int n_large = 0;
+    const int* d_res_prefix = nullptr;   // first residual slice of each tier-2 solve (+ total)
+    int res_total = 0;
     std::vector<Tier2Class> classes;
     hipStream_t side = nullptr;                     // tier 1 runs here, beside tier 2
     hipEvent_t fork = nullptr, join = nullptr;
@@ -517,6 +546,8 @@ static void clear_set(ph3_batch* b) {
     for (int s = 0; s < 3; ++s) { b->d_small[s] = nullptr; b->n_small[s] = 0; }
     b->d_large = nullptr;
     b->n_large = 0;
+    b->d_res_prefix = nullptr;
+    b->res_total = 0;
 }
 
 extern "C" size_t ph3_solve_desc_bytes(void) { return sizeof(ph3_solve_desc); }
@@ -544,8 +575,8 @@ extern "C" int ph3_batch_create(ph3_batch** out, int capacity) {
     const size_t cap = (size_t)capacity;
     hipError_t e = hipMalloc((void**)&b->d_desc, sizeof(ph3_solve_desc) * cap);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_partial, sizeof(double) * ph3::kSlices * cap);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_iarena, sizeof(int) * 4 * cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_partial, sizeof(double) * ph3::kMaxSlices * cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_iarena, sizeof(int) * (5 * cap + 1));
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_llarena, sizeof(long long) * (cap + kMaxClasses));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
@@ -578,7 +609,10 @@ static int check_desc(const ph3_solve_desc& d, int i) {
     if (d.storage < 0 || d.storage > 2) { set_error("solve %d: bad storage", i); return PH3_EINVAL; }
     if (!d.buf[0] || !d.buf[1] || d.buf[0] == d.buf[1]) { set_error("solve %d: buffers", i); return PH3_EINVAL; }
     if (d.mode != 0 && d.mode != 1) { set_error("solve %d: mode %d", i, d.mode); return PH3_EINVAL; }
-    if (d.tile < 0 || d.tile > 4) { set_error("solve %d: tile %d not in 0..4", i, d.tile); return PH3_EINVAL; }
+    if (d.tile < 0 || d.tile > 6 || (d.dims == 2 && d.tile > 4)) {
+        set_error("solve %d: tile %d not in 0..%d", i, d.tile, d.dims == 2 ? 4 : 6);
+        return PH3_EINVAL;
+    }
     if (d.kpass < 0 || d.kpass > 8 || (d.kpass & (d.kpass - 1))) {
         set_error("solve %d: kpass %d not in {0, 1, 2, 4, 8}", i, d.kpass);
         return PH3_EINVAL;
@@ -676,6 +710,10 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     size_t o_small[3], o_init[3];
     for (int st = 0; st < 3; ++st) { o_small[st] = put(small[st]); o_init[st] = put(init[st]); }
     const size_t o_large = put(large);
+    std::vector<int> res_prefix(large.size() + 1, 0);
+    for (size_t t = 0; t < large.size(); ++t)
+        res_prefix[t + 1] = res_prefix[t] + ph3::res_slices(table[large[t]]);
+    const size_t o_res = put(res_prefix);
     std::vector<size_t> o_list(classes.size()), o_prefix(classes.size());
     for (size_t ci = 0; ci < classes.size(); ++ci) {
         Tier2Class& c = classes[ci];
@@ -703,8 +741,9 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
             lh.push_back(lh.back() + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, sh,
                                                      c.tpg, c.max_chunk).tiles);
         }
+        c.tiles = lh.back();
     }
-    if (ih.size() > 4 * (size_t)b->cap || lh.size() > (size_t)b->cap + kMaxClasses) {
+    if (ih.size() > 5 * (size_t)b->cap + 1 || lh.size() > (size_t)b->cap + kMaxClasses) {
         set_error("ph3_batch_set: index tables exceed the batch's arenas");
         return PH3_EINVAL;
     }
@@ -744,6 +783,8 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     }
     b->n_large = (int)large.size();
     b->d_large = b->n_large ? b->d_iarena + o_large : nullptr;
+    b->d_res_prefix = b->d_iarena + o_res;
+    b->res_total = res_prefix.back();
     for (size_t ci = 0; ci < classes.size(); ++ci) {
         classes[ci].d_list = b->d_iarena + o_list[ci];
         classes[ci].d_prefix = b->d_llarena + o_prefix[ci];
@@ -768,6 +809,8 @@ static void pass_shape(const Tier2Class& c, const ph3_batch* b, int p, hipStream
             case 2: return launch_pass<T, P, S, 3, K, 2, 8, 4, kStoreMasked>(c, b, p, s);
             case 3: return launch_pass<T, P, S, 3, K, 1, 8, 8, kStoreMasked>(c, b, p, s);
             case 4: return launch_pass<T, P, S, 3, K, 2, 4, 8, kStoreMasked>(c, b, p, s);
+            case 5: return launch_pass<T, P, S, 3, K, 1, 8, 4, kStorePlanes>(c, b, p, s);
+            case 6: return launch_pass<T, P, S, 3, K, 1, 8, 8, kStorePlanes>(c, b, p, s);
             default: return launch_pass<T, P, S, 3, K, 1, 8, 4, kStoreMasked>(c, b, p, s);
         }
     } else {
@@ -838,7 +881,8 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
     // tier 2: segments of PH3_BATCH_CHECK varlen passes, then advance / residual / check
     if (b->n_large) {
         const unsigned gl = (unsigned)((b->n_large + 255) / 256);
-        const unsigned gr = (unsigned)(b->n_large * ph3::kSlices < ph3::kPassGroups ? b->n_large * ph3::kSlices : ph3::kPassGroups);
+        const int rt = b->res_total > 0 ? b->res_total : 1;
+        const unsigned gr = (unsigned)(rt < 2 * ph3::kPassGroups ? rt : 2 * ph3::kPassGroups);
         // Classes touch disjoint solves, so within a segment each runs its chain of passes on
         // its own stream (the first on `stream`), forked and joined with events: a round then
         // takes about the longest chain instead of the sum of all of them.
@@ -868,15 +912,15 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
             bool st_used[3] = {false, false, false};
             for (const auto& c : b->classes) st_used[c.storage] = true;
             for (int st = 0; st < 3; ++st) {
-                if (!st_used[st]) continue;
+                if (!st_used[st] || b->res_total == 0) continue;   // no tier-2 Jacobi solve
                 switch (st) {
-                    case ph3::kF32: ph3::batch_residual<float><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial); break;
-                    case ph3::kF16: ph3::batch_residual<_Float16><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial); break;
-                    default: ph3::batch_residual<ph3::bf16_t><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial); break;
+                    case ph3::kF32: ph3::batch_residual<float><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial); break;
+                    case ph3::kF16: ph3::batch_residual<_Float16><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial); break;
+                    default: ph3::batch_residual<ph3::bf16_t><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial); break;
                 }
                 PH3_LAUNCH_CHECK("batch residual");
             }
-            ph3::batch_check<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->n_large, b->d_partial);
+            ph3::batch_check<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial);
             PH3_LAUNCH_CHECK("batch check");
         }
     }

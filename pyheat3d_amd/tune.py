@@ -179,7 +179,7 @@ BATCH_CLASSES = {(2, 8): (65, 2048), (3, 2): (17, 127), (3, 4): (128, 256)}
 BATCH_POLICIES = {2: ((0, 0), (4096, 256), (8192, 256), (16384, 256), (4096, 128), (8192, 512)),
                   3: ((0, 0), (1024, 128), (2048, 128), (4096, 128), (8192, 128), (4096, 256),
                       (8192, 256))}
-BATCH_SHAPES = (1, 2, 3, 4)
+BATCH_SHAPES = {2: (1, 2, 3, 4), 3: (1, 2, 3, 4, 5, 6)}   # tier2_shape ids per dims
 
 
 def tuning_mix(n: int = 256, seed: int = 77, jacobi_cap: int = 2048, small: bool = False):
@@ -256,7 +256,7 @@ def tune_batch(n: int = 256, verbose: bool = True) -> list[dict[str, Any]]:
                 if score(t) < best:
                     best, best_t, policy = score(t), t, trial
             for h in (0, 1):
-                for sid in BATCH_SHAPES:
+                for sid in BATCH_SHAPES[c[0]]:
                     if sid == shapes[(c, h)]:
                         continue
                     trial = {**shapes, (c, h): sid}

@@ -217,7 +217,8 @@ def test_batch_table_entries_are_valid():
 
     table = tune.load_table()
     tiles = {k: e for k, e in table.items() if k.startswith("batch/tile/")}
-    assert tiles and all(0 <= e["tile"] <= 4 for e in tiles.values())
+    assert tiles and all(0 <= e["tile"] <= (4 if k.startswith("batch/tile/2d") else 6)
+                         for k, e in tiles.items())
     for (d, _k), (lo, hi) in tune.BATCH_CLASSES.items():
         for b in range(tune.bucket((lo,) * d), tune.bucket((hi,) * d) + 1):
             assert f"batch/tile/{d}d/b{b}" in tiles
