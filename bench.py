@@ -61,6 +61,7 @@ def time_variant(key: str, u0: np.ndarray, stencil, steps: int, warmup: int,
     t_up = time.perf_counter()
     st = make_stepper(key, u0, stencil)  # allocation + host-to-device upload of u0
     t_up = time.perf_counter() - t_up
+    tile = dict(st.tile)                  # the launch the library resolved (ph3_sweep_plan)
     try:
         k = var.steps_per_pass
         steps = max(k, steps - steps % k)
@@ -104,7 +105,7 @@ def time_variant(key: str, u0: np.ndarray, stencil, steps: int, warmup: int,
             "kernel_avg_ms": ev_ms / launches, "bytes_per_update": bpu,
             "algorithmic_bytes_per_launch": bpu * k * cells,
             "effective_GBs": cells * steps / wall * bpu / 1e9, "finite": finite,
-            "check": check,
+            "check": check, "tile": tile,
             "host_transfer": {"upload_s": round(t_up, 4), "download_s": round(t_down, 4),
                               "bytes_each_way": int(u0.nbytes),
                               "glups_including_transfers": cells * steps / (wall + t_up + t_down) / 1e9}}
@@ -449,7 +450,23 @@ def main() -> int:
         others[key] = {"glups": round(r["glups"], 3), "ms_per_step": round(r["ms_per_step"], 4),
                        "minimal_GBs": round(r["effective_GBs"], 1), "k": r["k"],
                        "single_step_equiv_GBs": round(r["glups"] * 8, 1),
+                       "tile": r["tile"],
                        "max_mode_amplitude_rel_err": max(r["check"]["mode_amplitude_rel_err"])}
+        if r["k"] > 1:   # the halo over-fetch model of the reference (traffic.py:38-50)
+            from pyheat3d_amd.traffic import halo_model_bytes
+
+            hb = halo_model_bytes(3, r["k"], 4, r["tile"])
+            others[key]["halo_model_bytes_per_unit"] = round(hb, 4)
+            others[key]["halo_model_GBs"] = round(r["glups"] * hb, 1)
+        pmc = load_traffic(f"{key.split('/')[1]}_{a.n}")
+        if pmc and pmc.get("variant") == key and pmc.get("n") == a.n:
+            # HBM bytes per launch from the committed counter runs, as a rate at this run's
+            # kernel time, and per site update
+            t = pmc["hbm_bytes_per_launch"]
+            others[key]["counter_traffic"] = {
+                "bytes_per_launch": round(t), "over_algorithmic": round(pmc["traffic_over_algorithmic"], 4),
+                "bytes_per_update": round(t / (a.n ** 3 * r["k"]), 4),
+                "GBs_at_this_run": round(t / (r["kernel_avg_ms"] * 1e-3) / 1e9, 1)}
     torch_cmp = None
     if not a.no_torch:
         torch_cmp = torch_eager_comparator(u0, stencil, 20, 3)

@@ -97,6 +97,25 @@ int ph3_sweep(int kind, int k, int storage, int periodic, int dims, int nz, int 
               float c0, float c1, const void* u, void* o, const float* b,
               const ph3_tile* tile, void* stream);
 
+/* The launch ph3_sweep would make for these arguments, without launching: the lane width,
+ * rows, waves and march chunk it resolves (tuned tile, buffer alignment, occupancy), the tile
+ * geometry and the workgroup count. This is the record behind the reference's `tile` field of
+ * temporal-blocking results (benchmark.py:222-231), which it reads from its tile dicts. `u`,
+ * `o` and `b` are the buffers the sweep would use (their alignment picks the lane width);
+ * nothing is read or written through them. */
+typedef struct ph3_plan {
+    int kind;            /* PH3_NAIVE, PH3_ZMARCH or PH3_TB, as launched (k = 1 runs the march) */
+    int k;               /* steps per launch */
+    int vec, rows, waves, zch;   /* sites per lane, rows per wave, waves per workgroup, march
+                                    chunk (planes in 3D, rows in 2D); 0 where not used */
+    int tile_x, tile_y;  /* extended tile: sites along x, rows along y (1 for a 2D strip) */
+    int out_x, out_y;    /* of which written: sites along x, rows along y */
+    int threads;         /* lanes per workgroup */
+    long long groups;    /* workgroups of the launch */
+} ph3_plan;
+int ph3_sweep_plan(int kind, int k, int storage, int periodic, int dims, int nz, int ny, int nx,
+                   const void* u, void* o, const float* b, const ph3_tile* tile, ph3_plan* plan);
+
 /* Steps per pass supported by kind PH3_TB for (dims, storage); nonzero if supported. */
 int ph3_tb_supported(int dims, int k);
 

@@ -33,6 +33,18 @@ class Tile(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+class Plan(ctypes.Structure):
+    """``ph3_plan``: the launch a sweep resolves (lane width, rows, waves, chunk, tile, groups)."""
+
+    _fields_ = [("kind", c_int), ("k", c_int), ("vec", c_int), ("rows", c_int),
+                ("waves", c_int), ("zch", c_int), ("tile_x", c_int), ("tile_y", c_int),
+                ("out_x", c_int), ("out_y", c_int), ("threads", c_int),
+                ("groups", ctypes.c_longlong)]
+
+    def as_dict(self) -> dict[str, int]:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 # name -> (restype, argtypes); every symbol of include/pyheat3d_hip.h
 SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ph3_last_error": (c_char_p, []),
@@ -60,6 +72,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "ph3_event_elapsed_ms": (c_int, [POINTER(c_float), c_void_p, c_void_p]),
     "ph3_sweep": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_float,
                           c_float, c_void_p, c_void_p, c_void_p, POINTER(Tile), c_void_p]),
+    "ph3_sweep_plan": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                               c_void_p, c_void_p, c_void_p, POINTER(Tile), POINTER(Plan)]),
     "ph3_tb_supported": (c_int, [c_int, c_int]),
     "ph3_encode": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "ph3_decode": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
@@ -243,3 +257,17 @@ def sweep(kind: int, k: int, storage: str, periodic: bool, shape: tuple[int, ...
                          c0, c1, u, o, b or None, ctypes.byref(tile) if tile else None,
                          stream or None)
     check(rc, "sweep")
+
+
+def sweep_plan(kind: int, k: int, storage: str, periodic: bool, shape: tuple[int, ...], u: int,
+               o: int, b: int | None, tile: Tile | None) -> dict[str, int]:
+    """The launch :func:`sweep` would make for these arguments (``ph3_sweep_plan``)."""
+    dims = len(shape)
+    nz, ny, nx = (1, *shape) if dims == 2 else shape
+    plan = Plan()
+    rc = lib().ph3_sweep_plan(kind, k, STORAGE_CODE[storage], int(periodic), dims, nz, ny, nx,
+                              u, o, b or None, ctypes.byref(tile) if tile else None,
+                              ctypes.byref(plan))
+    check(rc, "sweep_plan")
+    return plan.as_dict()
+

@@ -59,8 +59,14 @@ class HipStepper(Stepper):
 
             self.tile_override = lookup(self.kind, self.steps_per_pass, self.dims, self.storage,
                                         self.bc, self.shape, source is not None)
-        self.tile = {"kind": self.kind, "k": self.steps_per_pass,
-                     **(self.tile_override.as_dict() if self.tile_override else {})}
+        self.tile = self.plan()
+
+    def plan(self) -> dict:
+        """The launch of this stepper's main pass as the library resolves it (lane width, rows,
+        waves, march chunk, tile geometry, workgroups), and the tuned tile it started from."""
+        p = _hip.sweep_plan(self._kind_code, self.steps_per_pass, self.storage, self.periodic,
+                            self.shape, self.a, self.b, self.src or None, self.tile_override)
+        return {**p, "tuned": self.tile_override.as_dict() if self.tile_override else None}
 
     _allow_2d = False
 

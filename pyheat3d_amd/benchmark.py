@@ -37,10 +37,69 @@ PLANS: dict[str, dict[str, Any]] = {
 }
 
 
+_SMI: Any = None
+
+
+def _smi_handle():
+    """AMD SMI handle of GPU 0, or None (library absent or no access)."""
+    global _SMI
+    if _SMI is None:
+        try:
+            import amdsmi
+
+            amdsmi.amdsmi_init()
+            _SMI = (amdsmi, amdsmi.amdsmi_get_processor_handles()[0])
+        except Exception:
+            _SMI = False
+    return _SMI or None
+
+
+def gpu_state() -> dict[str, Any]:
+    """GPU clocks, temperature and power right now (AMD SMI; the reference reads the same
+    through NVML for its throttling audit, ``benchmark.py:86-100``). Empty when unavailable."""
+    smi = _smi_handle()
+    if smi is None:
+        return {}
+    amdsmi, h = smi
+    out: dict[str, Any] = {}
+
+    def clk(kind):
+        info = amdsmi.amdsmi_get_clock_info(h, kind)
+        return info.get("clk", info.get("cur_clk"))
+
+    for key, fn in (("gfx_clock_MHz", lambda: clk(amdsmi.AmdSmiClkType.GFX)),
+                    ("mem_clock_MHz", lambda: clk(amdsmi.AmdSmiClkType.MEM)),
+                    ("temp_hotspot_C", lambda: amdsmi.amdsmi_get_temp_metric(
+                        h, amdsmi.AmdSmiTemperatureType.HOTSPOT,
+                        amdsmi.AmdSmiTemperatureMetric.CURRENT)),
+                    ("power_W", lambda: (lambda p: p.get("current_socket_power",
+                                                         p.get("average_socket_power")))(
+                        amdsmi.amdsmi_get_power_info(h))),
+                    ("gfx_activity_pct", lambda: amdsmi.amdsmi_get_gpu_activity(h).get("gfx_activity"))):
+        try:
+            v = fn()
+            if v is not None and v != "N/A":
+                out[key] = v
+        except Exception:
+            pass
+    return out
+
+
+def cpu_model() -> str | None:
+    """The host CPU's model name (``lscpu``'s "Model name", read from /proc/cpuinfo)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or None
+
+
 def machine_info() -> dict[str, Any]:
     from . import _hip
 
-    info: dict[str, Any] = {"cpu": platform.processor() or None, "cpu_count": os.cpu_count(),
+    info: dict[str, Any] = {"cpu": cpu_model(), "cpu_count": os.cpu_count(),
                             "python": platform.python_version(), "numpy": np.__version__}
     try:
         d = _hip.device_info(0)
@@ -62,6 +121,7 @@ def time_variant(key: str, u0: np.ndarray, min_time: float, repeats: int,
     dims = u0.ndim
     st = make_stepper(key, u0, heat_stencil(dims, r if r is not None else 0.9 / (2 * dims), bc))
     k = var.steps_per_pass
+    tile = dict(st.tile)
     try:
         t0 = time.perf_counter()
         st.step(2 * k)
@@ -102,10 +162,14 @@ def time_variant(key: str, u0: np.ndarray, min_time: float, repeats: int,
         "pct_spec_bw": 100.0 * eff / traffic.SPEC_BW_GBS,
         "roofline_MLUPS": traffic.roofline_mlups(copy_bw or traffic.SPEC_BW_GBS, bpu),
         "flops_per_unit": traffic.flops_per_update(dims), "GFLOPs": med * traffic.flops_per_update(dims) / 1e3,
-        "warmup_s": warm, "finite": finite, "notes": var.description,
+        "warmup_s": warm, "finite": finite, "gpu_state": gpu_state(), "tile": tile,
+        "notes": var.description,
     }
     if k > 1:
         rec["single_step_equiv_GBs"] = med * traffic.minimal_bytes(var.bytes_per_value, 1) / 1e3
+        hb = traffic.halo_model_bytes(dims, k, var.bytes_per_value, tile)
+        rec["halo_model_bytes_per_unit"] = hb
+        rec["halo_model_GBs"] = med * hb / 1e3
     return rec
 
 

@@ -95,6 +95,11 @@ def _cmd_validate(a: argparse.Namespace) -> int:
         ok = 0.95 < x["order"] < 1.05
         bad += not ok
         print(f"temporal  {x['variant']:22s} {x['dims']}D order {x['order']:.3f} {'ok' if ok else 'FAIL'}")
+    for name in ("laplace", "poisson"):
+        for x in res[name]:
+            ok = 1.85 < x["order"] < 2.15
+            bad += not ok
+            print(f"{name:9s} {x['variant']:22s} {x['dims']}D order {x['order']:.3f} {'ok' if ok else 'FAIL'}")
     for x in res["laplace_quadratic"]:
         print(f"quadratic {x['variant']:22s} {x['dims']}D n={x['n']} {x['status']} after "
               f"{x['sweeps']} sweeps, err {x['err']:.2e}")

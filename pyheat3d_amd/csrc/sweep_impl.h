@@ -15,7 +15,17 @@ struct SweepArgs {
     const float* b;
     ph3_tile tile;
     hipStream_t stream;
+    ph3_plan* plan = nullptr;   // set: describe the launch in *plan instead of making it
 };
+
+// Fills *a.plan (when set) and returns true: the caller then returns without launching.
+inline bool planned(const SweepArgs& a, int kind, int k, int vec, int rows, int waves, int zch,
+                    int tile_x, int tile_y, int out_x, int out_y, int threads, long long groups) {
+    if (!a.plan) return false;
+    *a.plan = ph3_plan{kind, k, vec, rows, waves, zch, tile_x, tile_y, out_x, out_y, threads,
+                       groups};
+    return true;
+}
 
 int launch_naive(const SweepArgs& a);
 int launch_zmarch(const SweepArgs& a);

@@ -125,6 +125,10 @@ static int naive_launch(const SweepArgs& a) {
         ph3host::set_error("naive: grid too large (ny=%d nz=%d)", a.ny, a.nz);
         return PH3_EINVAL;
     }
+    if (planned(a, PH3_NAIVE, 1, VEC, kNaiveBY, 1, 0, kNaiveBX * VEC, kNaiveBY, kNaiveBX * VEC,
+                kNaiveBY, kNaiveBX * kNaiveBY,
+                (long long)gx * gy * (a.dims == 3 ? a.nz : 1)))
+        return PH3_OK;
     if (a.dims == 2) {
         naive2d<T, P, S, VEC><<<dim3(gx, gy), block, 0, a.stream>>>(
             (const T*)a.u, (T*)a.o, a.b, a.ny, a.nx, a.c0, a.c1);

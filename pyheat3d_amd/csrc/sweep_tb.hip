@@ -106,6 +106,8 @@ static int tb3_launch(const SweepArgs& a) {
     }
     const long groups = tiles * ((a.nz + zch - 1) / zch);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb3d: grid too large"); return PH3_EINVAL; }
+    if (planned(a, PH3_TB, K, V, R, W, zch, 64 * V, W * R, G::OX, OY, 64 * W, groups))
+        return PH3_OK;
     tb3d<T, P, S, V, R, W, K><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
         (const T*)a.u, (T*)a.o, a.b, a.nz, a.ny, a.nx, a.c0, a.c1, zch, tiles_x, tiles_y,
         groups % 8 == 0 ? 1 : 0);
@@ -127,6 +129,7 @@ static int tb2_launch(const SweepArgs& a) {
     }
     const long groups = (long)tiles_x * ((a.ny + ych - 1) / ych);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb2d: grid too large"); return PH3_EINVAL; }
+    if (planned(a, PH3_TB, K, V, 0, W, ych, 64 * V, 1, G::OX, 1, 64 * W, groups)) return PH3_OK;
     tb2d<T, P, S, V, K, W><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
         (const T*)a.u, (T*)a.o, a.b, a.ny, a.nx, a.c0, a.c1, ych, tiles_x,
         groups % 8 == 0 ? 1 : 0);

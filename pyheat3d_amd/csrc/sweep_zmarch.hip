@@ -82,6 +82,8 @@ static int zmarch_launch3(const SweepArgs& a) {
     if (groups > 0x7fffffffL) { ph3host::set_error("zmarch: grid too large"); return PH3_EINVAL; }
     const int remap = (groups % 8) == 0 ? 1 : 0;
     const int nt = (a.tile.flags & PH3_TILE_NT_STORE) ? 1 : 0;
+    if (planned(a, PH3_ZMARCH, 1, V, R, W, zch, 64 * V, R * W, 64 * V, R * W, 64 * W, groups))
+        return PH3_OK;
     zmarch3d<T, P, S, V, R, W><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
         (const T*)a.u, (T*)a.o, a.b, a.nz, a.ny, a.nx, a.c0, a.c1, zch, tiles_x, tiles_y, remap,
         nt);
@@ -103,6 +105,8 @@ static int ymarch_launch2(const SweepArgs& a) {
     const int chunks = (a.ny + ych - 1) / ych;
     const long groups = (long)tiles_x * chunks;
     const int remap = (groups % 8) == 0 ? 1 : 0;
+    if (planned(a, PH3_ZMARCH, 1, V, R, W, ych, 64 * V, 1, 64 * V, 1, 64 * W, groups))
+        return PH3_OK;
     ymarch2d<T, P, S, V, R, W><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
         (const T*)a.u, (T*)a.o, a.b, a.ny, a.nx, a.c0, a.c1, ych, tiles_x, remap);
     PH3_LAUNCH_CHECK("ymarch2d launch");

@@ -52,3 +52,16 @@ def tb_model_bytes(dims: int, K: int, bytes_per_value: int, tx: int, ty: int, zc
         over = tx / (tx - 2.0 * K)
     over *= (zch + 2.0 * K) / zch
     return bytes_per_value * (over + 1.0) / K
+
+
+def halo_model_bytes(dims: int, K: int, bytes_per_value: int, plan: dict) -> float:
+    """:func:`tb_model_bytes` for the tile a temporal-blocking launch actually uses (its
+    ``ph3_plan``): the extended tile ``tile_x`` x ``tile_y`` of which ``out_x`` x ``out_y`` is
+    written (the x halo is rounded up to whole lane vectors), and ``K`` extra planes (3D) or rows
+    (2D) at both ends of each ``zch`` march."""
+    over = plan["tile_x"] / plan["out_x"]
+    if dims == 3:
+        over *= plan["tile_y"] / plan["out_y"]
+    zch = plan["zch"]
+    over *= (zch + 2.0 * K) / zch
+    return bytes_per_value * (over + 1.0) / K

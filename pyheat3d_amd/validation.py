@@ -7,6 +7,9 @@ Restates the reference's ``src/pyheat3d/validation.py`` studies for float32 GPU 
 * :func:`spatial_convergence` / :func:`temporal_convergence` -- observed orders 2 and 1
   (``validation.py:67-86``); in float32 the finest grids approach round-off, so the studies
   use grids where the discretisation error dominates;
+* :func:`laplace_convergence` / :func:`poisson_convergence` -- Jacobi for the reference's
+  fixed sweep count (``jacobi_iterations``) on a smooth harmonic function and on a manufactured
+  Poisson problem: observed order 2 (``validation.py:89-147``);
 * :func:`laplace_quadratic` -- Jacobi to tolerance on a quadratic harmonic, which the 5/7-point
   stencil reproduces exactly (``validation.py:126-132``);
 * :func:`poisson_eigen` -- Jacobi on ``-lap u = d pi^2 prod sin(pi x_i)``, whose discrete
@@ -73,6 +76,76 @@ def temporal_convergence(key: str, dims: int, n: int, steps_list: list[int],
             "order": slope([x["dt"] for x in runs], [x["err_semidiscrete"] for x in runs])}
 
 
+def jacobi_iterations(n: int, tol: float) -> int:
+    """Sweeps for Jacobi (spectral radius cos(pi h)) to damp an O(1) error to ``tol``
+    (``validation.py:89-93``)."""
+    return int(math.ceil(math.log(tol) / math.log(math.cos(math.pi / (n - 1)))))
+
+
+def solve_fixed(key: str, exact: np.ndarray, tol: float,
+                source: np.ndarray | None = None) -> tuple[np.ndarray, int]:
+    """Jacobi from a zero interior with the boundary values of ``exact`` for the reference's
+    fixed (even) sweep count (``validation.py:96-110``), in the variant's float32 arithmetic."""
+    from .problem import jacobi_stencil
+
+    dims = exact.ndim
+    u0 = exact.astype(np.float32)
+    u0[(slice(1, -1),) * dims] = 0.0
+    sweeps = jacobi_iterations(exact.shape[0], tol)
+    sweeps += sweeps % 2
+    b = None if source is None else np.asarray(source, dtype=np.float32)
+    st = make_stepper(key, u0, jacobi_stencil(dims), b)
+    try:
+        st.step(sweeps)
+        u = st.get().astype(np.float64)
+    finally:
+        st.close()
+    return u, sweeps
+
+
+def _jacobi_orders(runs: list[dict[str, Any]]) -> dict[str, float]:
+    """Observed order over the grids where the discretisation error dominates float32.
+
+    Jacobi converges to the fixed point of its rounded map, at most eps32 / (1 - cos(pi h)) from
+    the exact one (the per-sweep rounding times the iteration's sensitivity; each run records it
+    as ``fp32_bound``); a grid where that bound reaches half the error has left the asymptotic
+    range in float32 (2D n = 129: 2e-4 against a 1.7e-5 error). ``order`` fits the other grids
+    (all grids if fewer than three remain), ``order_all_grids`` every grid.
+    """
+    for r in runs:
+        r["fp32_bound"] = 2.0 ** -24 / (1.0 - math.cos(math.pi / (r["n"] - 1)))
+    dom = [r for r in runs if r["fp32_bound"] < 0.5 * r["err"]]
+    if len(dom) < 3:
+        dom = runs
+    return {"order": slope([x["h"] for x in dom], [x["err"] for x in dom]),
+            "order_grids": [x["n"] for x in dom],
+            "order_all_grids": slope([x["h"] for x in runs], [x["err"] for x in runs])}
+
+
+def laplace_convergence(key: str, dims: int, ns: list[int], tol: float = 1e-10) -> dict[str, Any]:
+    """Converged Jacobi solution vs the smooth harmonic function (``analytic.harmonic_sinh``):
+    order 2 in h (``validation.py:113-123``)."""
+    runs = []
+    for n in ns:
+        ex = an.harmonic_sinh(n, dims)
+        u, it = solve_fixed(key, ex, tol)
+        runs.append({"n": n, "h": 1.0 / (n - 1), "iterations": it, "err": _rel(u, ex)})
+    return {"dims": dims, "variant": key, "tol": tol, "runs": runs, **_jacobi_orders(runs)}
+
+
+def poisson_convergence(key: str, dims: int, ns: list[int], tol: float = 1e-10) -> dict[str, Any]:
+    """Jacobi for ``lap u = f`` with the manufactured solution of ``analytic.poisson_manufactured``
+    and source ``b = -h^2 f / (2d)`` (rounded to float32): order 2 in h
+    (``validation.py:135-147``)."""
+    runs = []
+    for n in ns:
+        ex, f = an.poisson_manufactured(n, dims)
+        h = 1.0 / (n - 1)
+        u, it = solve_fixed(key, ex, tol, source=-h * h * f / (2 * dims))
+        runs.append({"n": n, "h": h, "iterations": it, "err": _rel(u, ex)})
+    return {"dims": dims, "variant": key, "tol": tol, "runs": runs, **_jacobi_orders(runs)}
+
+
 def laplace_quadratic(key: str, dims: int, n: int, tol: float = 1e-6) -> dict[str, Any]:
     ex = an.harmonic_quadratic(n, dims)
     u0 = ex.astype(np.float32)
@@ -121,6 +194,14 @@ def run_all(quick: bool = True) -> dict[str, Any]:
     out["spatial"] = [spatial_convergence(k, 2, ns2) for k in keys2] + \
                      [spatial_convergence(k, 3, ns3) for k in keys3]
     out["temporal"] = [temporal_convergence(k, 2, 17, [64, 128, 256]) for k in keys2[:1]]
+    # the reference's grids (validation.py:258-262); quick: the 2D studies on three grids
+    out["laplace"] = [laplace_convergence(k, 2, [9, 17, 33] if quick else [9, 17, 33, 65, 129])
+                      for k in keys2[:1]]
+    out["poisson"] = [poisson_convergence(k, 2, [9, 17, 33] if quick else [9, 17, 33, 65])
+                      for k in keys2[:1]]
+    if not quick:
+        out["laplace"] += [laplace_convergence(k, 3, [9, 17, 33, 65]) for k in keys3[:1]]
+        out["poisson"] += [poisson_convergence(k, 3, [9, 17, 33]) for k in keys3[:1]]
     out["laplace_quadratic"] = [laplace_quadratic(k, 2, 33) for k in keys2[:1]] + \
                                [laplace_quadratic(k, 3, 17) for k in keys3[:1]]
     out["poisson_eigen"] = [poisson_eigen(k, 33) for k in keys3]

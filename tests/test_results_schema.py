@@ -74,3 +74,30 @@ def test_covers_every_rung_at_every_size(bench):
         for dims in var.dims:
             for n in PLANS["dev"]["sizes"][dims]:
                 assert (key, (n,) * dims) in seen, (key, n, dims)
+
+
+def test_records_carry_tile_gpu_state_and_halo_model(bench):
+    """The reference's per-record tile, gpu_state and (temporal blocking) halo-model fields
+    (benchmark.py:216-231): the tile is the launch the library resolved (ph3_sweep_plan)."""
+    assert bench["machine"].get("cpu")
+    for r in bench["runs"]:
+        assert isinstance(r["gpu_state"], dict), r["key"]
+        t = r["tile"]
+        for f in ("vec", "waves", "zch", "tile_x", "out_x", "threads", "groups"):
+            assert f in t, (r["key"], f)
+        assert t["groups"] > 0 and t["out_x"] <= t["tile_x"]
+        if r["steps_per_pass"] > 1:
+            assert r["halo_model_bytes_per_unit"] > r["bytes_per_unit"], r["key"]
+            assert r["halo_model_GBs"] == pytest.approx(
+                r["median"] * r["halo_model_bytes_per_unit"] / 1e3)
+
+
+def test_halo_model_of_a_plan_is_the_reference_formula():
+    """With a halo of exactly K sites per side, the plan-based model is traffic.tb_model_bytes."""
+    from pyheat3d_amd import traffic
+
+    for dims, K, tx, ty, zch in ((3, 4, 64, 64, 128), (3, 2, 64, 32, 64), (2, 8, 64, 1, 128)):
+        plan = {"tile_x": tx, "out_x": tx - 2 * K, "tile_y": ty,
+                "out_y": ty - 2 * K if dims == 3 else 1, "zch": zch}
+        assert traffic.halo_model_bytes(dims, K, 4, plan) == pytest.approx(
+            traffic.tb_model_bytes(dims, K, 4, tx, ty, zch))
