@@ -273,7 +273,7 @@ class BatchSolver:
             self.close()
             raise
 
-    def run(self, max_rounds: int = 10 ** 6) -> dict:
+    def run(self, max_rounds: int = 10 ** 6, progress=None, every_s: float = 30.0) -> dict:
         """Replay rounds until every solve has finished; returns timing and round counts.
 
         Every request counts as submitted when the run starts; its completion time is the end
@@ -285,6 +285,7 @@ class BatchSolver:
         """
         graph = None
         t0 = time.perf_counter()
+        t_last = t0
         rounds = compactions = 0
         self.finish_s: list[float | None] = [None] * len(self.requests)
         try:
@@ -303,6 +304,9 @@ class BatchSolver:
                     if d.status != 0 and self.finish_s[row] is None:
                         self.finish_s[row] = t_round
                 running = [row for row, d in zip(self.table, descs) if d.status == 0]
+                if progress is not None and time.perf_counter() - t_last >= every_s:
+                    t_last = time.perf_counter()
+                    progress(rounds, len(running), t_last - t0)
                 if not running:
                     break
                 if self.compact and len(running) <= len(self.table) // 2:
@@ -344,7 +348,8 @@ class BatchSolver:
 
 def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
                      check_every: int = _hip.BATCH_CHECK, graphs: bool = False,
-                     kpass: dict[int, int] | None = None) -> tuple[list[Result], float]:
+                     kpass: dict[int, int] | None = None,
+                     progress=None) -> tuple[list[Result], float]:
     """Baseline: each request on its own stepper, one after another; one launch per pass (the
     reference's stepping), or with ``graphs`` the stepper's replayed graphs of 16 passes for
     grids of up to ``HipStepper.GRAPH_MAX_SITES`` sites.
@@ -396,6 +401,8 @@ def solve_one_by_one(requests: list[Request], keep_fields: bool = False,
             st.close()
         res.seconds = time.perf_counter() - t0
         out.append(res)
+        if progress is not None:
+            progress(len(out), len(requests), time.perf_counter() - t_all)
     return out, time.perf_counter() - t_all
 
 

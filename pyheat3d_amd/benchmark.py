@@ -275,14 +275,20 @@ def accuracy_sample(reqs, max_sites: int = 1 << 22, max_heat_work: float = 2e9,
     return out
 
 
-def accuracy_stats(reqs, results) -> dict[str, Any]:
+def accuracy_stats(reqs, results, progress: bool = False) -> dict[str, Any]:
     """Max and median error vs FP64 per (mode, outcome) over the results that carry a field."""
+    import time as _time
+
     from .batch import error_vs_fp64
 
     rows = []
+    t_last = _time.perf_counter()
     for q, r in zip(reqs, results):
         if r.u is None:
             continue
+        if progress and _time.perf_counter() - t_last > 30:
+            t_last = _time.perf_counter()
+            _printer("accuracy: {0} results compared with FP64 solutions")(len(rows))
         e = error_vs_fp64(q, r.u)
         r.extra["err_vs_fp64"] = e
         r.u = None
@@ -301,7 +307,8 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
                     n3: tuple[int, int] = (32, 256), frac_jacobi: float = 0.5,
                     storage: str = "float32", baseline_every: int = 8, round_sweeps: int = 256,
                     jacobi_cap: int = 10 ** 6, accuracy_every: int = 8,
-                    graphs_baseline: bool = True, use_graph: bool = True) -> dict[str, Any]:
+                    graphs_baseline: bool = True, use_graph: bool = True,
+                    progress: bool = False) -> dict[str, Any]:
     """All requests submitted at t = 0: batched graph rounds vs one launch per pass per solve.
 
     The per-solve leg runs every ``baseline_every``-th request (a systematic sample of the same
@@ -322,14 +329,19 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
     acc_ids = accuracy_sample(reqs, every=accuracy_every)
     bs = BatchSolver(reqs, round_sweeps=round_sweeps, use_graph=use_graph)
     try:
-        timing = bs.run()
+        timing = bs.run(progress=_printer("batch: round {0}, {1} running, {2:.0f} s")
+                        if progress else None)
         kp = batch_kpass(bs)
         res = bs.results(keep_fields=acc_ids)
     finally:
         bs.close()
     work = sum(q.sites * r.steps for q, r in zip(reqs, res))
     sample = reqs[::baseline_every]
-    base, _ = solve_one_by_one(sample, kpass=kp)
+    if progress:
+        _printer("batch: {0} solves in {1:.1f} s; per-solve leg on {2} of them")(
+            n, timing["seconds"], len(sample))
+    show = _printer("per-solve: {0}/{1} solves, {2:.0f} s") if progress else None
+    base, _ = solve_one_by_one(sample, kpass=kp, progress=show)
     base_time = sum(r.seconds for r in base)
     by_rid = {r.rid: r for r in res}
     agree = sum(1 for b in base if by_rid[b.rid].steps == b.steps and by_rid[b.rid].status == b.status)
@@ -347,18 +359,27 @@ def batch_benchmark(n: int = 2048, seed: int = 2024, n2: tuple[int, int] = (64, 
         "speedup": (n / timing["seconds"]) / (len(sample) / base_time),
         "baseline_agrees_on_steps_and_status": f"{agree}/{len(sample)}",
         "sample_sweeps": {int(b.rid): int(by_rid[b.rid].steps) for b in base},
-        "accuracy_vs_fp64": accuracy_stats(reqs, res),
+        "accuracy_vs_fp64": accuracy_stats(reqs, res, progress),
     }
     rs = out["batched"]["reached_target"]
     ps = out["per_solve"]["reached_target"]
     if ps["solves"] and rs["solves"]:
         out["speedup_reached_target"] = rs["solves_per_s"] / ps["solves_per_s"]
     if graphs_baseline:
-        based, _ = solve_one_by_one(sample, graphs=True, kpass=kp)
+        based, _ = solve_one_by_one(sample, graphs=True, kpass=kp, progress=show)
         based_time = sum(r.seconds for r in based)
         out["per_solve_graphs"] = {"seconds": based_time, "solves_per_s": len(sample) / based_time}
         out["speedup_vs_graphs"] = (n / timing["seconds"]) / (len(sample) / based_time)
     return out
+
+
+def _printer(fmt: str):
+    """A progress callback that prints one formatted line to stderr."""
+    import sys
+
+    def show(*args):
+        print(fmt.format(*args), file=sys.stderr, flush=True)
+    return show
 
 
 def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
@@ -366,7 +387,7 @@ def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
                     storage: str = "float32", budget_gb: float = 200.0,
                     round_sweeps: int = 128, jacobi_cap: int = 10 ** 6,
                     accuracy_every: int = 8, time_limit: float = 3600.0,
-                    use_graph: bool = True) -> dict[str, Any]:
+                    use_graph: bool = True, progress: bool = False) -> dict[str, Any]:
     """The mix arriving as a Poisson process of ``rate`` requests/s, served by continuous
     batching under an HBM budget; latency = completion - arrival, reported over all solves and
     per outcome, with a sample of results compared with FP64 NumPy solutions."""
@@ -379,7 +400,9 @@ def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
     srv = Server(int(budget_gb * 1e9), round_sweeps=round_sweeps, keep_fields=acc_ids,
                  use_graph=use_graph)
     try:
-        res, st = srv.serve(reqs, arr, time_limit=time_limit)
+        res, st = srv.serve(reqs, arr, time_limit=time_limit,
+                            progress=_printer("serve: round {0}, {1} finished, {2} resident, "
+                                              "{3:.0f} s") if progress else None)
     finally:
         srv.close()
     return {"workload": f"{n} Poisson arrivals at {rate:g}/s of the mix: 50% heat (200 steps) / "
@@ -390,4 +413,4 @@ def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
             "max_resident_GB": st.max_resident_bytes / 1e9,
             **outcome_stats(reqs, res, st.seconds),
             "statuses": {s_: sum(r.status == s_ for r in res) for s_ in ("done", "stagnated", "max_sweeps")},
-            "accuracy_vs_fp64": accuracy_stats(reqs, res)}
+            "accuracy_vs_fp64": accuracy_stats(reqs, res, progress)}

@@ -124,7 +124,8 @@ def _cmd_batch(a: argparse.Namespace) -> int:
 
     r = batch_benchmark(a.n, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
                         a.frac_jacobi, a.storage, a.baseline_every, a.round_sweeps,
-                        a.jacobi_cap, a.accuracy_every, use_graph=not a.no_graph)
+                        a.jacobi_cap, a.accuracy_every, use_graph=not a.no_graph,
+                        graphs_baseline=not a.no_graphs_baseline, progress=True)
     print(json.dumps(r, indent=1))
     return 0
 
@@ -134,7 +135,7 @@ def _cmd_serve(a: argparse.Namespace) -> int:
 
     r = serve_benchmark(a.n, a.rate, a.seed, (a.n2_min, a.n2_max), (a.n3_min, a.n3_max),
                         a.storage, a.budget_gb, a.round_sweeps, a.jacobi_cap, a.accuracy_every,
-                        use_graph=not a.no_graph)
+                        use_graph=not a.no_graph, progress=True)
     print(json.dumps(r, indent=1))
     return 0
 
@@ -228,6 +229,8 @@ def main(argv: list[str] | None = None) -> int:
     ba.add_argument("--n", type=int, default=2048)
     ba.add_argument("--frac-jacobi", type=float, default=0.5)
     ba.add_argument("--baseline-every", type=int, default=8)
+    ba.add_argument("--no-graphs-baseline", action="store_true",
+                    help="skip the per-solve leg on the stepper's replayed graphs")
     _size_args(ba)
     ba.set_defaults(fn=_cmd_batch)
 

@@ -74,9 +74,10 @@ class Server:
         self.batch.close()
         self.stream.close()
 
-    def serve(self, requests: list[Request], arrivals: np.ndarray,
-              time_limit: float = 600.0) -> tuple[list[Result], ServeStats]:
-        """Serve ``requests`` arriving at ``arrivals`` (seconds after start); blocking."""
+    def serve(self, requests: list[Request], arrivals: np.ndarray, time_limit: float = 600.0,
+              progress=None, every_s: float = 30.0) -> tuple[list[Result], ServeStats]:
+        """Serve ``requests`` arriving at ``arrivals`` (seconds after start); blocking.
+        ``progress(rounds, finished, resident, seconds)`` is called every ``every_s`` seconds."""
         apply_tuned_tiles(requests)
         order = np.argsort(arrivals, kind="stable")
         pending = [(float(arrivals[i]), requests[i]) for i in order]
@@ -91,6 +92,7 @@ class Server:
         dirty = True
         rounds = max_res = max_bytes = 0
         t0 = time.perf_counter()
+        t_last = t0
         try:
             while pending or resident:
                 now = time.perf_counter() - t0
@@ -151,6 +153,9 @@ class Server:
                     results[req.rid] = res
                     dirty = True
                 resident, descs = keep_r, keep_d
+                if progress is not None and time.perf_counter() - t_last >= every_s:
+                    t_last = time.perf_counter()
+                    progress(rounds, len(results), len(resident), t_last - t0)
         finally:
             if graph is not None:
                 graph.close()
