@@ -4,7 +4,8 @@ Workload (the headline roofline configuration): 3D 7-point explicit FTCS heat, 1
 periodic boundaries, initial state = sum of three sine modes (wavenumbers 1, 2, 3 per axis, unit
 amplitude) + 1e-3 * N(0,1) noise from seed 1, r = 0.15. One "step" is one sweep of the field
 (one lattice update of all 1024^3 sites) by the headline variant ``hip-cdna4/zmarch``; the other
-rungs (naive, tb<k>) are timed on the same field and reported beside it.
+rungs (naive, tb<k>) are timed on the same field and reported beside it, each over at least
+``--rung-steps`` steps (a tb4 run of 20 steps would be 5 launches, too few to time).
 
     python bench.py [--steps K] [--warmup W] [--n 1024] [--variant hip-cdna4/zmarch]
 
@@ -394,6 +395,9 @@ def main() -> int:
     ap.add_argument("--variant", default="hip-cdna4/zmarch")
     ap.add_argument("--others", default="hip-cdna4/naive,hip-cdna4/tb2,hip-cdna4/tb3,hip-cdna4/tb4",
                     help="comma-separated rungs timed beside the headline ('' for none)")
+    ap.add_argument("--rung-steps", type=int, default=240,
+                    help="the other rungs run max(--steps, this) steps after max(--warmup, 24), "
+                         "so that a rung of k steps per launch is timed over enough launches")
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-min-time", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -445,7 +449,7 @@ def main() -> int:
         if not okk:
             others[key] = {"unavailable": whyk}
             continue
-        r = time_variant(key, u0, stencil, a.steps, a.warmup, amp0)
+        r = time_variant(key, u0, stencil, max(a.steps, a.rung_steps), max(a.warmup, 24), amp0)
         log(f"{key}: {r['glups']:.2f} GLUPS ({r['effective_GBs']:.0f} GB/s minimal model)")
         others[key] = {"glups": round(r["glups"], 3), "ms_per_step": round(r["ms_per_step"], 4),
                        "minimal_GBs": round(r["effective_GBs"], 1), "k": r["k"],

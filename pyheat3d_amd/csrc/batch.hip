@@ -24,7 +24,7 @@ using ph3host::set_error;
 namespace ph3 {
 
 constexpr int kWG = 256;          // lanes per workgroup in every batch kernel
-constexpr int kPassGroups = 2048; // workgroups of the persistent / looping launches
+constexpr int kPassGroups = 2048; // workgroups of the persistent tier-1 / residual launches
 constexpr int kMinSlices = 16;    // residual slices of a tier-2 Jacobi grid: at least ...
 constexpr int kMaxSlices = 256;   // ... and at most these, one per kSliceSites sites
 constexpr int kSliceSites = 32768;
@@ -60,7 +60,8 @@ struct Prob {
     long long steps;
     __device__ explicit Prob(const ph3_solve_desc& d)
         : dims(d.dims), nz(d.nz), ny(d.ny), nx(d.nx), mode(d.mode), P(d.periodic != 0),
-          c0(d.c0), c1(d.c1), src(d.src), b0(d.buf[0]), b1(d.buf[1]), steps(d.steps) {}
+          c0(d.c0), c1(d.c1), src(d.src ? as_global(d.src) : nullptr), b0(as_global(d.buf[0])),
+          b1(as_global(d.buf[1])), steps(d.steps) {}
     __device__ void* buf(long long i) const { return (i & 1) ? b1 : b0; }
 };
 
@@ -130,34 +131,74 @@ __device__ void wg_sweep(const T* __restrict__ u, T* __restrict__ o, const float
     }
 }
 
-// Sum over rows row0, row0 + rstride, ... of the squared defect, per lane, in FP64.
+// Sum of the squared defect over the sites lane, lane + 64, ... of rows row0, row0 + rstride,
+// ..., per lane, in FP64, in that order. The (row, 64-site block) items are taken kResU at a
+// time and all their loads issued before the first is used (one item at a time, each wave waited
+// for its 5-7 loads per site block). Sites off the row or on a fixed Dirichlet boundary add an
+// exact 0, which leaves the sum as if they were skipped (it is never -0). row0 and rstride must
+// be uniform over the wave.
+constexpr int kResU = 4;
+
 template <typename T>
 __device__ double residual_rows(const T* __restrict__ u, const float* __restrict__ b, int dims,
                                 int nz, int ny, int nx, bool P, float c0, float c1, int row0,
-                                int rstride, int lane0, int lstride) {
+                                int rstride, int lane) {
     const double c0d = c0, c1d = c1;
     const size_t plane = (size_t)nx * ny;
     const int rows = nz * ny;
+    const int nxb = (nx + 63) / 64;   // 64-site blocks per row
+    int row = __builtin_amdgcn_readfirstlane(row0), xb = 0;   // the next item
     double acc = 0.0;
-    for (int row = row0; row < rows; row += rstride) {
-        const int k = row / ny, j = row - k * ny;
-        if (!P && (j == 0 || j == ny - 1 || (dims == 3 && (k == 0 || k == nz - 1)))) continue;
-        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-        const size_t base = (size_t)k * plane, r0 = base + (size_t)j * nx;
-        for (int i = lane0; i < nx; i += lstride) {
-            if (!P && (i == 0 || i == nx - 1)) continue;
-            const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-            double s = (double)ld(u, r0 + im) + (double)ld(u, r0 + ip);
-            s += (double)ld(u, base + (size_t)jm * nx + i) + (double)ld(u, base + (size_t)jp * nx + i);
+    while (row < rows) {
+        size_t ic[kResU], xm[kResU], xp[kResU], ym[kResU], yp[kResU], zm[kResU], zp[kResU];
+        bool ok[kResU];
+#pragma unroll
+        for (int q = 0; q < kResU; ++q) {
+            const bool live = row < rows;
+            const int r = live ? row : 0;
+            const int k = r / ny, j = r - k * ny, i = xb * 64 + lane;
+            ok[q] = live && i < nx &&
+                    (P || (j > 0 && j < ny - 1 && i > 0 && i < nx - 1 &&
+                           (dims == 2 || (k > 0 && k < nz - 1))));
+            const int ii = ok[q] ? i : 0;
+            const int im = ii == 0 ? nx - 1 : ii - 1, ip = ii == nx - 1 ? 0 : ii + 1;
+            const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+            const size_t base = (size_t)k * plane, r0 = base + (size_t)j * nx;
+            ic[q] = r0 + ii;
+            xm[q] = r0 + im;
+            xp[q] = r0 + ip;
+            ym[q] = base + (size_t)jm * nx + ii;
+            yp[q] = base + (size_t)jp * nx + ii;
             if (dims == 3) {
                 const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
-                s += (double)ld(u, (size_t)km * plane + (size_t)j * nx + i) +
-                     (double)ld(u, (size_t)kp * plane + (size_t)j * nx + i);
+                zm[q] = (size_t)km * plane + (size_t)j * nx + ii;
+                zp[q] = (size_t)kp * plane + (size_t)j * nx + ii;
+            } else {
+                zm[q] = zp[q] = ic[q];
             }
-            const double uc = (double)ld(u, r0 + i);
+            if (live && ++xb == nxb) { xb = 0; row += rstride; }
+        }
+        float vc[kResU], vxm[kResU], vxp[kResU], vym[kResU], vyp[kResU], vzm[kResU], vzp[kResU],
+            vb[kResU];
+#pragma unroll
+        for (int q = 0; q < kResU; ++q) {
+            vc[q] = ld(u, ic[q]);
+            vxm[q] = ld(u, xm[q]);
+            vxp[q] = ld(u, xp[q]);
+            vym[q] = ld(u, ym[q]);
+            vyp[q] = ld(u, yp[q]);
+            if (dims == 3) { vzm[q] = ld(u, zm[q]); vzp[q] = ld(u, zp[q]); }
+            vb[q] = b ? b[ic[q]] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < kResU; ++q) {
+            double s = (double)vxm[q] + (double)vxp[q];
+            s += (double)vym[q] + (double)vyp[q];
+            if (dims == 3) s += (double)vzm[q] + (double)vzp[q];
+            const double uc = (double)vc[q];
             double r = c0d * uc + c1d * s - uc;
-            if (b) r += (double)b[r0 + i];
-            acc += r * r;
+            if (b) r += (double)vb[q];
+            acc += ok[q] ? r * r : 0.0;
         }
     }
     return acc;
@@ -181,7 +222,7 @@ template <typename T>
 __device__ double wg_residual(const Prob& d, const void* u, double* sm) {
     const double part = residual_rows<T>((const T*)u, d.src, d.dims, d.nz, d.ny, d.nx, d.P,
                                          d.c0, d.c1, threadIdx.x >> 6, kWG / 64,
-                                         threadIdx.x & 63, 64);
+                                         threadIdx.x & 63);
     return wg_sum(part, sm);
 }
 
@@ -297,7 +338,7 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 
 // Tier-2 tile shapes. Single steps: the z-march tile (3D, 4 waves x 4 rows, 64 sites wide) or
 // the y-march tile (2D, 4 strips of 64 columns). Fused steps (float32, K > 1), by shape id
-// (ph3_solve_desc.tile; 0 = the library's rule, see default_tile):
+// (ph3_solve_desc.tile; 0 = the library's rule, see tier2_shape_for):
 //   3D  1: 1-site lanes, 4 waves x 8 rows (64 x 32 extended tile, 256 lanes)
 //       2: 2-site lanes, 4 waves x 8 rows (128 x 32)
 //       3: 1-site lanes, 8 waves x 8 rows (64 x 64, 512 lanes: the single-grid tb4 shape)
@@ -305,7 +346,7 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 //       5, 6: shapes 1 and 3 storing through buffers on the stored planes only (StoreMode)
 //   2D  1: 1-site lanes, masked stores        2: 2-site lanes, masked stores
 //       3: 1-site lanes, a store every step   4: 2-site lanes, a store every step
-//       (4 strips per tile; store modes as in tb_tiles.h StoreMode)
+//       (one strip per wave, 4 waves per workgroup; store modes as in tb_tiles.h StoreMode)
 // A wider shape recomputes a smaller share of halo sites; the narrow ones split small grids
 // into more tiles. Results never depend on the shape.
 struct T2Shape {
@@ -339,12 +380,15 @@ __host__ __device__ inline T2Shape tier2_shape(int dims, int K, int id) {
 // [lo, hi] planes or rows. Results do not depend on the tiling.
 struct T2Geom {
     int tx, ty, chunk;  // tiles along x, along y (1 in 2D), march chunk
-    long long tiles;
+    long long tiles;    // workgroup tiles of one pass over the grid (they set the chunk)
+    int strips;         // 2D fused passes: strips of one row chunk (one wave each)
+    long long items;    // work items of the pass kernel: 2D fused, strips x chunks; else tiles
 };
 
 __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, int K, int few,
                                              const T2Shape& sh, int tpg, int max_chunk) {
     T2Geom g;
+    g.strips = 0;
     if (dims == 3) {
         if (K <= 1) {
             g.tx = (nx + 63) / 64;
@@ -359,7 +403,8 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
     } else {
         const int kv = ((K + sh.v - 1) / sh.v) * sh.v;
         const int ox = K <= 1 ? 64 : 64 * sh.v - 2 * kv;
-        g.tx = ((nx + ox - 1) / ox + sh.w - 1) / sh.w;
+        g.strips = (nx + ox - 1) / ox;
+        g.tx = (g.strips + sh.w - 1) / sh.w;
         g.ty = 1;
     }
     const int march = dims == 3 ? nz : ny;
@@ -374,23 +419,34 @@ __host__ __device__ inline T2Geom tier2_geom(int dims, int nz, int ny, int nx, i
     if (ch < lo) ch = lo;
     if (ch > hi) ch = hi;
     g.chunk = ch;
-    g.tiles = txy * ((march + ch - 1) / ch);
+    const long long chunks = (march + ch - 1) / ch;
+    g.tiles = txy * chunks;
+    g.items = dims == 2 && K > 1 ? (long long)g.strips * chunks : g.tiles;
     return g;
 }
 
+// Work items of a tier-2 class per workgroup: a 2D fused pass gives each wave its own strip
+// (its waves are independent, so no wave of a workgroup idles beside a partial tile); the other
+// passes give each workgroup one tile.
+__host__ __device__ constexpr int items_per_group(int dims, int K, int W) {
+    return dims == 2 && K > 1 ? W : 1;
+}
+
 // One pass of a tier-2 class: V sites per lane, R rows per wave (3D fused), W waves, store mode
-// SM, as tier2_shape gives them for the class's shape id.
+// SM, as tier2_shape gives them for the class's shape id. The launch covers every work item
+// (prefix[n] of them) once: item i goes to workgroup i / items_per_group (a grid-stride loop
+// past 2^20 workgroups), so the hardware dispatcher balances the tiles.
 template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM>
 __global__ void __launch_bounds__(64 * W)
-batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
+batch_pass(const ph3_solve_desc* __restrict__ descs, const int* __restrict__ list,
            const long long* __restrict__ prefix, int n, int p, int few, int tpg,
            int max_chunk) {
     const T2Shape sh{V, R, W, SM};
+    constexpr int IPG = items_per_group(DIMS, K, W);
     const long long total = prefix[n];
-    // Tiles are dealt round-robin by workgroup index. (Dealing each XCD a contiguous block of
-    // every window of gridDim.x tiles, so neighbours share an L2, measured slower: full-range
-    // bench mix 100 -> 79 solves/s.)
-    for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    // wave-uniform item offset inside the workgroup (scalar registers)
+    const int sub = IPG > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+    for (long long t = (long long)blockIdx.x * IPG + sub; t < total; t += (long long)gridDim.x * IPG) {
         int lo = 0, hi = n - 1;  // last i with prefix[i] <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -400,29 +456,34 @@ batch_pass(const ph3_solve_desc* descs, const int* __restrict__ list,
         if (d.status == 0 && d.done + (long long)p * K < d.steps) {
             const long long local = t - prefix[lo];
             const int cur = (int)((d.done / K + p) & 1);
-            const T* u = (const T*)d.buf[cur];
-            T* o = (T*)d.buf[cur ^ 1];
+            const T* u = as_global((const T*)d.buf[cur]);
+            T* o = as_global((T*)d.buf[cur ^ 1]);
+            const float* src = S ? as_global(d.src) : nullptr;
             const T2Geom g = tier2_geom(DIMS, d.nz, d.ny, d.nx, K, few, sh, tpg, max_chunk);
-            const int ix = (int)(local % g.tx);
-            const long long rest = local / g.tx;
             if constexpr (DIMS == 3) {
+                const int ix = (int)(local % g.tx);
+                const long long rest = local / g.tx;
                 const int iy = (int)(rest % g.ty), iz = (int)(rest / g.ty);
                 if constexpr (K == 1)
-                    zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
+                    zmarch3d_tile<T, P, S, 1, kR3, kW3>(u, o, src, d.nz, d.ny, d.nx, d.c0,
                                                         d.c1, g.chunk, ix, iy, iz, 0);
                 else
-                    tb3d_tile<T, P, S, V, R, W, K, SM>(u, o, d.src, d.nz, d.ny, d.nx, d.c0,
+                    tb3d_tile<T, P, S, V, R, W, K, SM>(u, o, src, d.nz, d.ny, d.nx, d.c0,
                                                       d.c1, g.chunk, ix, iy, iz);
             } else {
                 if constexpr (K == 1)
-                    ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, d.src, d.ny, d.nx, d.c0, d.c1,
-                                                        g.chunk, ix, (int)rest);
+                    ymarch2d_tile<T, P, S, 1, kR2, kW2>(u, o, src, d.ny, d.nx, d.c0, d.c1,
+                                                        g.chunk, (int)(local % g.tx),
+                                                        (int)(local / g.tx));
                 else
-                    tb2d_tile<T, P, S, V, K, W, SM>(u, o, d.src, d.ny, d.nx, d.c0, d.c1,
-                                                   g.chunk, ix, (int)rest);
+                    tb2d_tile<T, P, S, V, K, SM>(u, o, src, d.ny, d.nx, d.c0, d.c1, g.chunk,
+                                                 (int)(local % g.strips),
+                                                 (int)(local / g.strips));
             }
         }
-        __syncthreads();  // the 3D tiles' LDS slot is reused by the next tile
+        // the 3D tiles' and the y-march's LDS slots are reused by the next tile (2D fused
+        // strips use no LDS, and their waves go on independently)
+        if constexpr (IPG == 1) __syncthreads();
     }
 }
 
@@ -445,7 +506,7 @@ __host__ __device__ inline int res_slices(const ph3_solve_desc& d) {
 // Slice `item` of the listed solves: `prefix` holds each solve's first slice (res_slices).
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list,
+batch_residual(const ph3_solve_desc* __restrict__ descs, const int* __restrict__ list,
                const int* __restrict__ prefix, int n, double* __restrict__ partial) {
     __shared__ double sm[kWG];
     const int total = prefix[n];
@@ -460,10 +521,11 @@ batch_residual(const ph3_solve_desc* descs, const int* __restrict__ list,
         if (d.mode != 1 || d.status != 0 || d.storage != Codec<T>::code) continue;
         // the state the segment's passes have written (the counters advance in batch_check)
         const int cur = (int)((done_after_segment(d) / kpass_of(d)) & 1);
-        const double part = residual_rows<T>((const T*)d.buf[cur], d.src, d.dims, d.nz,
+        const double part = residual_rows<T>(as_global((const T*)d.buf[cur]),
+                                             d.src ? as_global(d.src) : nullptr, d.dims, d.nz,
                                              d.ny, d.nx, d.periodic != 0, d.c0, d.c1,
                                              sl * (kWG / 64) + (threadIdx.x >> 6),
-                                             nsl * (kWG / 64), threadIdx.x & 63, 64);
+                                             nsl * (kWG / 64), threadIdx.x & 63);
         const double s = wg_sum(part, sm);
         if (threadIdx.x == 0) partial[item] = s;
     }
@@ -498,7 +560,7 @@ struct Tier2Class {
     int few = 0;   // tier2_geom's short-chunk mode (set from the class's tile count)
     int tpg = ph3::kTier2Tiles;  // tiles per grid the chunk aims at
     int max_chunk = 0;           // longest march chunk (0: the geometry's default)
-    long long tiles = 0;         // tiles of one pass over every grid of the class
+    long long items = 0;         // work items of one pass over every grid of the class
     std::vector<int> idx;
     const int* d_list = nullptr;          // into the batch's int arena
     const long long* d_prefix = nullptr;  // into the batch's prefix arena
@@ -643,16 +705,18 @@ static int choose_kpass(const ph3_solve_desc& d) {
 
 // Tile shape of a tier-2 solve (tier2_shape ids). The caller's `tile` is taken when it is one of
 // the fused shapes; 2-site lanes need an even width above 64 and 8-byte aligned buffers, and
-// fall back to the 1-site shape of the same width class otherwise. Default (tile 0): 2-site
-// lanes for float32 3D 2-step grids where allowed (a 128-wide tile then idles no more lanes
-// than two 64-wide ones), the narrow 1-site shape otherwise.
+// fall back to the 1-site shape of the same width class otherwise. Default (tile 0): in 3D,
+// 2-site lanes for 2-step grids where allowed (a 128-wide tile then idles no more lanes than
+// two 64-wide ones), and for 4-step grids the narrow 1-site shape storing whole planes (3D
+// 128..256 Jacobi mix: 1132 GLUPS with masked stores, 1212); in 2D, 2-site lanes storing on
+// every step (2D 600..2048 Jacobi mix: 2070 GLUPS with 1-site masked strips, 2512).
 static int tier2_shape_for(const ph3_solve_desc& d) {
     if (d.kpass <= 1) return 1;
     auto al8 = [](const void* q) { return ((uintptr_t)q & 7u) == 0; };
     const bool two = d.nx > 64 && d.nx % 2 == 0 && al8(d.buf[0]) && al8(d.buf[1]) &&
                      (!d.src || al8(d.src));
     int id = d.tile;
-    if (id == 0) id = d.dims == 3 && d.kpass == 2 ? 2 : 1;
+    if (id == 0) id = d.dims == 3 ? (d.kpass == 2 ? 2 : 5) : 4;
     if (!two && (id == 2 || id == 4)) id -= 1;
     return id;
 }
@@ -739,9 +803,9 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         for (int i : c.idx) {
             const ph3_solve_desc& d = table[i];
             lh.push_back(lh.back() + ph3::tier2_geom(d.dims, d.nz, d.ny, d.nx, d.kpass, c.few, sh,
-                                                     c.tpg, c.max_chunk).tiles);
+                                                     c.tpg, c.max_chunk).items);
         }
-        c.tiles = lh.back();
+        c.items = lh.back();
     }
     if (ih.size() > 5 * (size_t)b->cap + 1 || lh.size() > (size_t)b->cap + kMaxClasses) {
         set_error("ph3_batch_set: index tables exceed the batch's arenas");
@@ -796,7 +860,14 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
 
 template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM>
 static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
-    ph3::batch_pass<T, P, S, DIMS, K, V, R, W, SM><<<(unsigned)ph3::kPassGroups, 64 * W, 0, s>>>(
+    // one workgroup per items_per_group items (a multiple of 8 workgroups, one per XCD in
+    // turn), at most 2^20 (the kernel loops past that)
+    constexpr int ipg = ph3::items_per_group(DIMS, K, W);
+    long long g = (c.items + ipg - 1) / ipg;
+    g = (g + 7) / 8 * 8;
+    if (g > (1 << 20)) g = 1 << 20;
+    if (g < 8) g = 8;
+    ph3::batch_pass<T, P, S, DIMS, K, V, R, W, SM><<<(unsigned)g, 64 * W, 0, s>>>(
         b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p, c.few, c.tpg, c.max_chunk);
 }
 

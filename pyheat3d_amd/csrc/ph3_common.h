@@ -51,6 +51,23 @@ template <> struct Codec<bf16_t> {
     }
 };
 
+// A pointer that a kernel reads from device memory (a solve descriptor) is a generic pointer to
+// the compiler, and accesses through it become flat instructions. Flat loads count in both the
+// vector-memory and the LDS/scalar wait counters, so every wait for an LDS access, and the wait
+// for each prefetched row, then waits for all loads in flight (the batched tile passes ran at
+// half the speed of the same tile bodies launched with kernel-argument pointers). Re-deriving
+// the pointer from a global-address-space one lets the compiler emit global instructions; the
+// value is also made wave-uniform (scalar registers), which buffer resources need. The pointer
+// must be uniform over the wave.
+template <typename T>
+__device__ __forceinline__ T* as_global(T* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    using GT = __attribute__((address_space(1))) T;
+    return (T*)(GT*)(((unsigned long long)hi << 32) | lo);
+}
+
 template <typename T>
 __device__ __forceinline__ float ld(const T* __restrict__ p, size_t i) { return Codec<T>::load(p[i]); }
 

@@ -41,8 +41,8 @@ tb2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
      int ny, int nx, float c0, float c1, int ych, int tiles_x, int remap)
 {
     const int L = tb_logical_tile(remap);
-    tb2d_tile<T, P, S, V, K, W, kStoreAll>(u, o, b, ny, nx, c0, c1, ych, L % tiles_x,
-                                           L / tiles_x);
+    tb2d_tile<T, P, S, V, K, kStoreAll>(u, o, b, ny, nx, c0, c1, ych,
+                                        (L % tiles_x) * W + (int)(threadIdx.x >> 6), L / tiles_x);
 }
 
 // ---- launch ---------------------------------------------------------------------------------

@@ -266,17 +266,16 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     }
 }
 
-// Column group gx (W strips) of row chunk cy; waves are independent (no LDS, no barrier).
-// SM: the StoreMode of tb3d_tile (kStorePlanes skips the stores of rows that are not stored).
-template <typename T, bool P, bool S, int V, int K, int W, int SM = kStoreMasked>
+// Strip `strip` (64*V extended columns) of row chunk cy, marched by one wave; waves are
+// independent (no LDS, no barrier). SM: the StoreMode of tb3d_tile (kStorePlanes skips the
+// stores of rows that are not stored).
+template <typename T, bool P, bool S, int V, int K, int SM = kStoreMasked>
 __device__ __forceinline__ void
 tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
-          int ny, int nx, float c0, float c1, int ych, int gxg, int cy)
+          int ny, int nx, float c0, float c1, int ych, int strip, int cy)
 {
     using G = TbGeom<V, K>;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int strip = gxg * W + wave;
     const int y0 = cy * ych, y1 = min(y0 + ych, ny);
     const int gx = strip * G::OX - G::KV + lane * V;
     if (strip * G::OX - G::KV >= nx) return;          // whole wave past the grid (no barriers)
