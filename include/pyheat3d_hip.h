@@ -204,9 +204,10 @@ typedef struct ph3_solve_desc {
                                     3: 1-site lanes, 64 rows; 4: 2-site lanes, 32 rows in 8
                                     waves; 5/6: shapes 1/3 with buffer stores on the stored
                                     planes only. 2D 1/2: 1/2-site lanes; 3/4: the same with a store
-                                    on every step. 2-site shapes need an even width above 64
-                                    and 8-byte aligned buffers (else the 1-site shape). Never
-                                    changes the result. */
+                                    on every step; 5: shape 4 with each site loaded and stored
+                                    on its own (any width). Shapes 2 and 4 need an even width
+                                    above 64 and 8-byte aligned buffers (else the 1-site shape).
+                                    Never changes the result. */
     long long steps;             /* mode 0: steps; mode 1: sweep cap */
     double tol;                  /* mode 1: target ||r|| / ||r0|| */
     double log_rate;             /* mode 1: ln(Jacobi spectral radius) * PH3_BATCH_CHECK (< 0) */

@@ -346,11 +346,14 @@ batch_init_residual(ph3_solve_desc* descs, const int* __restrict__ list, int n) 
 //       5, 6: shapes 1 and 3 storing through buffers on the stored planes only (StoreMode)
 //   2D  1: 1-site lanes, masked stores        2: 2-site lanes, masked stores
 //       3: 1-site lanes, a store every step   4: 2-site lanes, a store every step
+//       5: 2-site lanes, a store every step, each site loaded and stored on its own (any
+//          width; tb_tiles.h tb2d_tile EW)
 //       (one strip per wave, 4 waves per workgroup; store modes as in tb_tiles.h StoreMode)
 // A wider shape recomputes a smaller share of halo sites; the narrow ones split small grids
 // into more tiles. Results never depend on the shape.
 struct T2Shape {
     int v, r, w, sm;   // sites per lane, rows per wave (3D), waves per workgroup, store mode
+    bool ew = false;   // 2D: sites loaded and stored one by one
 };
 
 __host__ __device__ inline T2Shape tier2_shape(int dims, int K, int id) {
@@ -369,6 +372,7 @@ __host__ __device__ inline T2Shape tier2_shape(int dims, int K, int id) {
         case 2: return T2Shape{2, 0, kW2, kStoreMasked};
         case 3: return T2Shape{1, 0, kW2, kStoreAll};
         case 4: return T2Shape{2, 0, kW2, kStoreAll};
+        case 5: return T2Shape{2, 0, kW2, kStoreAll, true};
         default: return T2Shape{1, 0, kW2, kStoreMasked};
     }
 }
@@ -436,12 +440,12 @@ __host__ __device__ constexpr int items_per_group(int dims, int K, int W) {
 // SM, as tier2_shape gives them for the class's shape id. The launch covers every work item
 // (prefix[n] of them) once: item i goes to workgroup i / items_per_group (a grid-stride loop
 // past 2^20 workgroups), so the hardware dispatcher balances the tiles.
-template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM>
+template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM, bool EW = false>
 __global__ void __launch_bounds__(64 * W)
 batch_pass(const ph3_solve_desc* __restrict__ descs, const int* __restrict__ list,
            const long long* __restrict__ prefix, int n, int p, int few, int tpg,
            int max_chunk) {
-    const T2Shape sh{V, R, W, SM};
+    const T2Shape sh{V, R, W, SM, EW};
     constexpr int IPG = items_per_group(DIMS, K, W);
     const long long total = prefix[n];
     // wave-uniform item offset inside the workgroup (scalar registers)
@@ -476,7 +480,7 @@ batch_pass(const ph3_solve_desc* __restrict__ descs, const int* __restrict__ lis
                                                         g.chunk, (int)(local % g.tx),
                                                         (int)(local / g.tx));
                 else
-                    tb2d_tile<T, P, S, V, K, SM>(u, o, src, d.ny, d.nx, d.c0, d.c1, g.chunk,
+                    tb2d_tile<T, P, S, V, K, SM, EW>(u, o, src, d.ny, d.nx, d.c0, d.c1, g.chunk,
                                                  (int)(local % g.strips),
                                                  (int)(local / g.strips));
             }
@@ -671,8 +675,8 @@ static int check_desc(const ph3_solve_desc& d, int i) {
     if (d.storage < 0 || d.storage > 2) { set_error("solve %d: bad storage", i); return PH3_EINVAL; }
     if (!d.buf[0] || !d.buf[1] || d.buf[0] == d.buf[1]) { set_error("solve %d: buffers", i); return PH3_EINVAL; }
     if (d.mode != 0 && d.mode != 1) { set_error("solve %d: mode %d", i, d.mode); return PH3_EINVAL; }
-    if (d.tile < 0 || d.tile > 6 || (d.dims == 2 && d.tile > 4)) {
-        set_error("solve %d: tile %d not in 0..%d", i, d.tile, d.dims == 2 ? 4 : 6);
+    if (d.tile < 0 || d.tile > 6 || (d.dims == 2 && d.tile > 5)) {
+        set_error("solve %d: tile %d not in 0..%d", i, d.tile, d.dims == 2 ? 5 : 6);
         return PH3_EINVAL;
     }
     if (d.kpass < 0 || d.kpass > 8 || (d.kpass & (d.kpass - 1))) {
@@ -704,20 +708,25 @@ static int choose_kpass(const ph3_solve_desc& d) {
 }
 
 // Tile shape of a tier-2 solve (tier2_shape ids). The caller's `tile` is taken when it is one of
-// the fused shapes; 2-site lanes need an even width above 64 and 8-byte aligned buffers, and
-// fall back to the 1-site shape of the same width class otherwise. Default (tile 0): in 3D,
+// the fused shapes; 2-site vector lanes need an even width above 64 and 8-byte aligned buffers,
+// and fall back otherwise: 2D shape 4 to shape 5 (site-by-site access), the others to the
+// 1-site shape of the same width class. Default (tile 0): in 3D,
 // 2-site lanes for 2-step grids where allowed (a 128-wide tile then idles no more lanes than
 // two 64-wide ones), and for 4-step grids the narrow 1-site shape storing whole planes (3D
 // 128..256 Jacobi mix: 1132 GLUPS with masked stores, 1212); in 2D, 2-site lanes storing on
-// every step (2D 600..2048 Jacobi mix: 2070 GLUPS with 1-site masked strips, 2512).
+// every step (2D 600..2048 Jacobi mix: 2070 GLUPS with 1-site masked strips, 2512), as one
+// vector access per lane where the width allows, else site by site (shape 5).
 static int tier2_shape_for(const ph3_solve_desc& d) {
     if (d.kpass <= 1) return 1;
     auto al8 = [](const void* q) { return ((uintptr_t)q & 7u) == 0; };
     const bool two = d.nx > 64 && d.nx % 2 == 0 && al8(d.buf[0]) && al8(d.buf[1]) &&
                      (!d.src || al8(d.src));
     int id = d.tile;
-    if (id == 0) id = d.dims == 3 ? (d.kpass == 2 ? 2 : 5) : 4;
-    if (!two && (id == 2 || id == 4)) id -= 1;
+    if (id == 0) id = d.dims == 3 ? (d.kpass == 2 ? 2 : 5) : (two ? 4 : 5);
+    if (!two) {
+        if (d.dims == 2 && id == 4) id = 5;               // the same lanes, site by site
+        else if (id == 2 || id == 4) id -= 1;
+    }
     return id;
 }
 
@@ -858,7 +867,7 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     return PH3_OK;
 }
 
-template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM>
+template <typename T, bool P, bool S, int DIMS, int K, int V, int R, int W, int SM, bool EW = false>
 static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStream_t s) {
     // one workgroup per items_per_group items (a multiple of 8 workgroups, one per XCD in
     // turn), at most 2^20 (the kernel loops past that)
@@ -867,7 +876,7 @@ static void launch_pass(const Tier2Class& c, const ph3_batch* b, int p, hipStrea
     g = (g + 7) / 8 * 8;
     if (g > (1 << 20)) g = 1 << 20;
     if (g < 8) g = 8;
-    ph3::batch_pass<T, P, S, DIMS, K, V, R, W, SM><<<(unsigned)g, 64 * W, 0, s>>>(
+    ph3::batch_pass<T, P, S, DIMS, K, V, R, W, SM, EW><<<(unsigned)g, 64 * W, 0, s>>>(
         b->d_desc, c.d_list, c.d_prefix, (int)c.idx.size(), p, c.few, c.tpg, c.max_chunk);
 }
 
@@ -889,6 +898,7 @@ static void pass_shape(const Tier2Class& c, const ph3_batch* b, int p, hipStream
             case 2: return launch_pass<T, P, S, 2, K, 2, 0, kW2, kStoreMasked>(c, b, p, s);
             case 3: return launch_pass<T, P, S, 2, K, 1, 0, kW2, kStoreAll>(c, b, p, s);
             case 4: return launch_pass<T, P, S, 2, K, 2, 0, kW2, kStoreAll>(c, b, p, s);
+            case 5: return launch_pass<T, P, S, 2, K, 2, 0, kW2, kStoreAll, true>(c, b, p, s);
             default: return launch_pass<T, P, S, 2, K, 1, 0, kW2, kStoreMasked>(c, b, p, s);
         }
     }

@@ -81,6 +81,13 @@ __device__ __forceinline__ float update(float c0, float c1, float u, float s) {
     return a + m;
 }
 
+// Index of the calling wave in its workgroup, in a scalar register: conditions on it are then
+// uniform branches or scalar selects instead of exec-masked blocks (the compiler cannot tell
+// that threadIdx.x >> 6 is the same in every lane of a wave).
+__device__ __forceinline__ int wave_id() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // Whole-wave lane shifts by one lane with DPP (wave_shr:1 / wave_shl:1, GFX9-family controls
 // available on gfx950): a VALU move instead of an LDS-crossbar permute. Lane 0 (resp. 63) has
 // no source lane; with bound_ctrl it receives 0 and no register has to be zeroed first.

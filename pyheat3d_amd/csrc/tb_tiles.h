@@ -61,7 +61,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     constexpr int OY = W * R - 2 * K;  // output rows per tile
     __shared__ float edge[2][W][2][64 * V];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = wave_id();
     const int z0 = cz * zch, z1 = min(z0 + zch, nz);
     const int gx = tx * G::OX - G::KV + lane * V;   // first x site of this lane (may be outside)
     const int gy0 = ty * OY - K + wave * R;         // first row of this wave (may be outside)
@@ -268,18 +268,25 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 
 // Strip `strip` (64*V extended columns) of row chunk cy, marched by one wave; waves are
 // independent (no LDS, no barrier). SM: the StoreMode of tb3d_tile (kStorePlanes skips the
-// stores of rows that are not stored).
-template <typename T, bool P, bool S, int V, int K, int SM = kStoreMasked>
+// stores of rows that are not stored). EW (float32 only): each of a lane's V sites is loaded
+// and stored on its own (V dword accesses instead of one V-wide access), so V-site lanes run
+// on rows of any width and alignment (odd widths would otherwise take 1-site lanes, whose
+// strips write 48 of 64 columns at k = 8 instead of 112 of 128).
+template <typename T, bool P, bool S, int V, int K, int SM = kStoreMasked, bool EW = false>
 __device__ __forceinline__ void
 tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
           int ny, int nx, float c0, float c1, int ych, int strip, int cy)
 {
+    static_assert(!EW || sizeof(T) == 4, "element-wise lanes are float32");
     using G = TbGeom<V, K>;
     const int lane = threadIdx.x & 63;
     const int y0 = cy * ych, y1 = min(y0 + ych, ny);
     const int gx = strip * G::OX - G::KV + lane * V;
     if (strip * G::OX - G::KV >= nx) return;          // whole wave past the grid (no barriers)
-    const int lx = vec_col<V>(gx, nx, P);
+    const int lx = EW ? 0 : vec_col<V>(gx, nx, P);
+    int lxe[V];                                       // EW: column of each site, wrapped / clamped
+#pragma unroll
+    for (int e = 0; e < V; ++e) lxe[e] = EW ? wrap_or_clamp(gx + e, nx, P) : 0;
     uint32_t fixmask = 0, wmask = 0;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -287,9 +294,27 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         const int col = lane * V + e;
         const bool fixed = !P && (x <= 0 || x >= nx - 1);
         if (fixed) fixmask |= 1u << e;
-        // whole lanes, fixed sites included (written back unchanged), as in tb3d_tile
+        // fixed sites included (written back unchanged), as in tb3d_tile; without EW a lane's
+        // V bits are all set or all clear (nx % V == 0)
         if (col >= G::KV && col < 64 * V - G::KV && x >= 0 && x < nx) wmask |= 1u << e;
     }
+    // one row of stored values (or of the source) at row offset ro, as a lane's V floats
+    auto load_row = [&](const T* __restrict__ p, size_t ro, float (&out)[V]) {
+        if constexpr (EW) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) out[e] = Codec<T>::load(p[ro + lxe[e]]);
+        } else {
+            load_vec<T, V>(p, ro + lx, out);
+        }
+    };
+    auto load_src = [&](size_t ro, float (&out)[V]) {
+        if constexpr (EW) {
+#pragma unroll
+            for (int e = 0; e < V; ++e) out[e] = b[ro + lxe[e]];
+        } else {
+            load_vec<float, V>(b, ro + lx, out);
+        }
+    };
     const bool wave_fix = !P && __any(fixmask != 0u);
     // row queues rotated by slot as in tb3d_tile (slots i, i+1, i+2 mod 3 hold rows c-1, c,
     // c+1 of level s at march step i)
@@ -309,9 +334,9 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         for (int s = 0; s < K; ++s)
 #pragma unroll
             for (int e = 0; e < V; ++e) bq[s][e] = 0.f;
-        load_vec<float, V>(b, (size_t)wrap_or_clamp(y0 - K - 1, ny, P) * nx + lx, bnext);
+        load_src((size_t)wrap_or_clamp(y0 - K - 1, ny, P) * nx, bnext);
     }
-    if constexpr (kDirect) load_vec<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx, pref);
+    if constexpr (kDirect) load_row(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx, pref);
     else prefp = load_packed<T, V>(u, (size_t)wrap_or_clamp(y0 - K, ny, P) * nx + lx);
     if constexpr (SM != kStoreMasked) {   // loop entry like the back edge (see arrived())
         if constexpr (kDirect) arrived(pref);
@@ -330,9 +355,9 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             unpack<T, V>(prefp, q[0][(k + 2) % 3]);
         }
         if (iy + 1 < y1 + K) {
-            const size_t ro = (size_t)wrap_or_clamp(iy + 1, ny, P) * nx + lx;
-            if constexpr (kDirect) load_vec<T, V>(u, ro, pref);
-            else prefp = load_packed<T, V>(u, ro);
+            const size_t ro = (size_t)wrap_or_clamp(iy + 1, ny, P) * nx;
+            if constexpr (kDirect) load_row(u, ro, pref);
+            else prefp = load_packed<T, V>(u, ro + lx);
         }
         if constexpr (S) {
 #pragma unroll
@@ -341,7 +366,7 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                 for (int e = 0; e < V; ++e) bq[s][e] = bq[s - 1][e];
 #pragma unroll
             for (int e = 0; e < V; ++e) bq[0][e] = bnext[e];
-            load_vec<float, V>(b, (size_t)wrap_or_clamp(iy, ny, P) * nx + lx, bnext);
+            load_src((size_t)wrap_or_clamp(iy, ny, P) * nx, bnext);
         }
         float res[V];
 #pragma unroll
@@ -371,16 +396,30 @@ tb2d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                     if (yfixed || ((fixmask >> e) & 1u)) dst[e] = ctr[e];
             }
         }
-        // wmask is all V bits or none
         const int c = iy - K;
         const bool yok = c >= y0 && c < y1 && (P || (c > 0 && c < ny - 1));
         if constexpr (SM == kStoreMasked) {
-            if (wmask && yok) store_vec<T, V>(o, (size_t)c * nx + gx, res);
+            if constexpr (EW) {
+#pragma unroll
+                for (int e = 0; e < V; ++e)
+                    if (yok && ((wmask >> e) & 1u)) st(o, (size_t)c * nx + gx + e, res[e]);
+            } else {
+                if (wmask && yok) store_vec<T, V>(o, (size_t)c * nx + gx, res);
+            }
         } else if (SM == kStoreAll || yok) {
             // kStoreAll: a store on every step, dropped outside the output rows and lanes
             const __amdgpu_buffer_rsrc_t ob =
                 buffer_over(o + (yok ? (size_t)c * nx : 0), yok ? (uint32_t)(nx * sizeof(T)) : 0u);
-            store_vec_buf<T, V>(ob, wmask ? (uint32_t)(gx * sizeof(T)) : kDropOffset, res);
+            if constexpr (EW) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    const float one[1] = {res[e]};
+                    store_vec_buf<T, 1>(ob, ((wmask >> e) & 1u) ? (uint32_t)((gx + e) * sizeof(T))
+                                                                : kDropOffset, one);
+                }
+            } else {
+                store_vec_buf<T, V>(ob, wmask ? (uint32_t)(gx * sizeof(T)) : kDropOffset, res);
+            }
         }
     }
     }

@@ -176,10 +176,11 @@ def batch_policy() -> list[tuple[int, int, int, int]]:
 #: (2D: 8-step passes from 4097 sites; 3D: 2-step passes below 128^3 sites, 4-step from there)
 BATCH_CLASSES = {(2, 8): (65, 2048), (3, 2): (17, 127), (3, 4): (128, 256)}
 #: (target_tiles, max_chunk) candidates per dims; (0, 0) is the library's default
-BATCH_POLICIES = {2: ((0, 0), (4096, 256), (8192, 256), (16384, 256), (4096, 128), (8192, 512)),
-                  3: ((0, 0), (1024, 128), (2048, 128), (4096, 128), (8192, 128), (4096, 256),
-                      (8192, 256))}
-BATCH_SHAPES = {2: (1, 2, 3, 4), 3: (1, 2, 3, 4, 5, 6)}   # tier2_shape ids per dims
+BATCH_POLICIES = {2: ((0, 0), (1024, 256), (2048, 256), (4096, 256), (8192, 256), (2048, 128),
+                      (4096, 128), (8192, 512)),
+                  3: ((0, 0), (512, 128), (1024, 128), (2048, 128), (4096, 128), (8192, 128),
+                      (2048, 256), (4096, 256))}
+BATCH_SHAPES = {2: (1, 2, 3, 4, 5), 3: (1, 2, 3, 4, 5, 6)}   # tier2_shape ids per dims
 
 
 def tuning_mix(n: int = 256, seed: int = 77, jacobi_cap: int = 2048, small: bool = False):
@@ -208,42 +209,73 @@ def _half(req, c) -> int:
     return 1 if bucket(req.shape) > (b0 + b1) // 2 else 0
 
 
-def _time_mixes(policy: dict, shapes: dict, n: int, reps: int = 2) -> tuple[float, float]:
+def tail_mix(seed: int = 91):
+    """The solves that set the length of a full mixed run: Jacobi Laplace on the largest grids
+    (2D N in 600..2048, 3D N in 128..256, two 2D solves per 3D one), which need 10^5-10^6
+    sweeps; timed over a fixed number of rounds, not to completion."""
+    from .batch import Request
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(48):
+        dims = 3 if i % 3 == 2 else 2
+        lo, hi = (600, 2048) if dims == 2 else (128, 256)
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        out.append(Request(shape=(n,) * dims, mode="jacobi", steps=10 ** 6,
+                           seed=int(rng.integers(1, 2 ** 62)), rid=i))
+    return out
+
+
+def _time_mixes(policy: dict, shapes: dict, n: int, reps: int = 2,
+                tail_rounds: int = 3) -> tuple[float, float, float]:
+    """Seconds of the full-range and small-grid tuning mixes run to completion (best of
+    ``reps``) and of ``tail_rounds`` rounds of the tail mix (after one untimed round)."""
     from .batch import BatchSolver
 
+    def shaped(reqs):
+        for r in reqs:
+            c = _class_of(r)
+            r.tile = shapes.get((c, _half(r, c)), 0) if c else 0   # 0: the library's rule
+        return reqs
+
+    params = [(d, k, t, ch) for (d, k), (t, ch) in policy.items()]
     out = []
     for small in (False, True):
         best = None
         for _ in range(reps):
-            reqs = tuning_mix(n, small=small)
-            for r in reqs:
-                c = _class_of(r)
-                r.tile = shapes.get((c, _half(r, c)), 0) if c else 0   # 0: the library's rule
-            params = [(d, k, t, ch) for (d, k), (t, ch) in policy.items()]
-            b = BatchSolver(reqs, round_sweeps=256, params=params, tuned_tiles=False)
+            b = BatchSolver(shaped(tuning_mix(n, small=small)), round_sweeps=256, params=params,
+                            tuned_tiles=False)
             try:
                 sec = b.run()["seconds"]
             finally:
                 b.close()
             best = sec if best is None else min(best, sec)
         out.append(best)
-    return out[0], out[1]
+    b = BatchSolver(shaped(tail_mix()), round_sweeps=256, params=params, tuned_tiles=False,
+                    compact=False)
+    try:
+        b.run(max_rounds=1)
+        out.append(b.run(max_rounds=tail_rounds)["seconds"])
+    finally:
+        b.close()
+    return out[0], out[1], out[2]
 
 
 def tune_batch(n: int = 256, verbose: bool = True) -> list[dict[str, Any]]:
-    """Autotune the batch's tier-2 passes on two mixes (:func:`tuning_mix`, full size ranges
-    and small grids): coordinate descent over each fused class's chunk policy (target tiles per
-    class, longest chunk) and the tile shape of the lower and upper half of its size range,
-    timing whole batch runs (best of 2). The score is the sum of both mixes' times relative to
-    the library's defaults. Returns table entries for ``update_batch_table``."""
+    """Autotune the batch's tier-2 passes on three mixes (:func:`tuning_mix`, full size ranges
+    and small grids, run to completion; :func:`tail_mix`, the largest Jacobi solves, a few
+    rounds): coordinate descent over each fused class's chunk policy (target tiles per class,
+    longest chunk) and the tile shape of the lower and upper half of its size range. The score
+    is the sum of the three mixes' times relative to the library's defaults. Returns table
+    entries for ``update_batch_table``."""
     policy = {c: (0, 0) for c in BATCH_CLASSES}
     shapes: dict = {(c, h): 0 for c in BATCH_CLASSES for h in (0, 1)}
     base = _time_mixes(policy, shapes, n)
-    score = lambda t: t[0] / base[0] + t[1] / base[1]   # noqa: E731
-    best_t, best = base, 2.0
+    score = lambda t: sum(x / y for x, y in zip(t, base))   # noqa: E731
+    best_t, best = base, float(len(base))
     if verbose:
         print(f"  batch tuning mixes ({n} solves), library defaults: {base[0]:.3f} s (full), "
-              f"{base[1]:.3f} s (small)", flush=True)
+              f"{base[1]:.3f} s (small), {base[2]:.3f} s (tail rounds)", flush=True)
     for sweep in range(2):
         for c in BATCH_CLASSES:
             for cand in BATCH_POLICIES[c[0]]:
@@ -252,7 +284,8 @@ def tune_batch(n: int = 256, verbose: bool = True) -> list[dict[str, Any]]:
                 trial = {**policy, c: cand}
                 t = _time_mixes(trial, shapes, n)
                 if verbose:
-                    print(f"  sweep {sweep} class {c} policy {cand}: {t[0]:.3f} / {t[1]:.3f} s", flush=True)
+                    print(f"  sweep {sweep} class {c} policy {cand}: {t[0]:.3f} / {t[1]:.3f} / "
+                          f"{t[2]:.3f} s", flush=True)
                 if score(t) < best:
                     best, best_t, policy = score(t), t, trial
             for h in (0, 1):
@@ -262,11 +295,13 @@ def tune_batch(n: int = 256, verbose: bool = True) -> list[dict[str, Any]]:
                     trial = {**shapes, (c, h): sid}
                     t = _time_mixes(policy, trial, n)
                     if verbose:
-                        print(f"  sweep {sweep} class {c} half {h} shape {sid}: {t[0]:.3f} / {t[1]:.3f} s", flush=True)
+                        print(f"  sweep {sweep} class {c} half {h} shape {sid}: {t[0]:.3f} / "
+                              f"{t[1]:.3f} / {t[2]:.3f} s", flush=True)
                     if score(t) < best:
                         best, best_t, shapes = score(t), t, trial
     if verbose:
-        print(f"  tuned: {best_t[0]:.3f} s (full), {best_t[1]:.3f} s (small)", flush=True)
+        print(f"  tuned: {best_t[0]:.3f} s (full), {best_t[1]:.3f} s (small), {best_t[2]:.3f} s "
+              f"(tail rounds)", flush=True)
     entries = []
     for c, (lo, hi) in BATCH_CLASSES.items():
         d, k = c

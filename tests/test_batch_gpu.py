@@ -184,8 +184,8 @@ def test_every_tier2_tile_shape_and_chunk_policy_matches_oracle(tile):
     reqs = []
     for sh in ((130, 150), (131, 149), (40, 36, 70), (33, 34, 67), (130, 128, 128)):
         for bc in ("dirichlet", "periodic"):
-            # 3D-only shapes 5 and 6: 2D solves take their library default
-            t2 = tile if len(sh) == 3 or tile <= 4 else 0
+            # 3D-only shape 6: 2D solves take their library default
+            t2 = tile if len(sh) == 3 or tile <= 5 else 0
             reqs.append(Request(shape=sh, mode="heat", steps=40, bc=bc, seed=700 + len(reqs),
                                 rid=len(reqs), tile=t2))
     src = (rng.random((40, 36, 70), dtype=np.float32) * np.float32(1e-3)).astype(np.float32)
