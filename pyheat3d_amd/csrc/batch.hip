@@ -568,6 +568,9 @@ struct Tier2Class {
     std::vector<int> idx;
     const int* d_list = nullptr;          // into the batch's int arena
     const long long* d_prefix = nullptr;  // into the batch's prefix arena
+    const int* d_res_prefix = nullptr;    // first residual slice of each solve (+ total), int arena
+    int res_total = 0;                    // residual slices of the class's Jacobi solves
+    int partial_base = 0;                 // the class's first residual partial
 };
 
 // Upper bound on tier-2 classes: storage (3) x dims (2) x bc (2) x source (2) x kpass (4) x
@@ -580,19 +583,14 @@ struct ph3_batch {
     int* d_counter = nullptr;
     double* d_partial = nullptr;
     // Per-set index tables live in two arenas sized for `cap` solves at creation: every list
-    // of a set (tier-1 lists, initial-residual lists, the tier-2 list, the per-class lists, the
-    // residual slice prefix) together holds at most 5 * cap + 1 ints, and the per-class tile
-    // prefixes at most cap + kMaxClasses long longs; the residual partials kMaxSlices * cap
-    // doubles. A set rewrites them in place and never frees device
+    // of a set (tier-1 lists, initial-residual lists, the per-class lists and residual slice
+    // prefixes) together holds at most 5 * cap + 1 ints, and the per-class tile prefixes at
+    // most cap + kMaxClasses long longs; the residual partials kMaxSlices * cap doubles. A set rewrites them in place and never frees device
     // memory, so a graph recorded from an earlier round never points at freed memory.
     int* d_iarena = nullptr;
     long long* d_llarena = nullptr;
     const int* d_small[3] = {nullptr, nullptr, nullptr};  // tier-1 lists per storage
     int n_small[3] = {0, 0, 0};
-    const int* d_large = nullptr;                         // every tier-2 solve
-    int n_large = 0;
-    const int* d_res_prefix = nullptr;   // first residual slice of each tier-2 solve (+ total)
-    int res_total = 0;
     std::vector<Tier2Class> classes;
     hipStream_t side = nullptr;                     // tier 1 runs here, beside tier 2
     hipEvent_t fork = nullptr, join = nullptr;
@@ -610,10 +608,6 @@ static void clear_set(ph3_batch* b) {
     b->n = 0;
     b->classes.clear();
     for (int s = 0; s < 3; ++s) { b->d_small[s] = nullptr; b->n_small[s] = 0; }
-    b->d_large = nullptr;
-    b->n_large = 0;
-    b->d_res_prefix = nullptr;
-    b->res_total = 0;
 }
 
 extern "C" size_t ph3_solve_desc_bytes(void) { return sizeof(ph3_solve_desc); }
@@ -754,14 +748,13 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     clear_set(b);
     std::vector<ph3_solve_desc> table(descs, descs + n);
     for (auto& d : table) d.kpass = choose_kpass(d);
-    std::vector<int> small[3], init[3], large;
+    std::vector<int> small[3], init[3];
     std::vector<Tier2Class> classes;
     for (int i = 0; i < n; ++i) {
         const ph3_solve_desc& d = table[i];
         init[d.storage].push_back(i);
         const long long sites = (long long)d.nz * d.ny * d.nx;
         if (sites <= PH3_BATCH_SMALL_SITES) { small[d.storage].push_back(i); continue; }
-        large.push_back(i);
         const int shape = tier2_shape_for(d);
         Tier2Class* c = nullptr;
         for (auto& k : classes)
@@ -782,12 +775,8 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
     auto put = [&ih](const std::vector<int>& v) { const size_t o = ih.size(); ih.insert(ih.end(), v.begin(), v.end()); return o; };
     size_t o_small[3], o_init[3];
     for (int st = 0; st < 3; ++st) { o_small[st] = put(small[st]); o_init[st] = put(init[st]); }
-    const size_t o_large = put(large);
-    std::vector<int> res_prefix(large.size() + 1, 0);
-    for (size_t t = 0; t < large.size(); ++t)
-        res_prefix[t + 1] = res_prefix[t] + ph3::res_slices(table[large[t]]);
-    const size_t o_res = put(res_prefix);
-    std::vector<size_t> o_list(classes.size()), o_prefix(classes.size());
+    std::vector<size_t> o_list(classes.size()), o_prefix(classes.size()), o_res(classes.size());
+    int partials = 0;
     for (size_t ci = 0; ci < classes.size(); ++ci) {
         Tier2Class& c = classes[ci];
         const ph3::T2Shape sh = ph3::tier2_shape(c.dims, c.kpass, c.shape);
@@ -815,6 +804,13 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
                                                      c.tpg, c.max_chunk).items);
         }
         c.items = lh.back();
+        // residual slices of the class's solves, and where its partial sums start
+        std::vector<int> rp(c.idx.size() + 1, 0);
+        for (size_t t = 0; t < c.idx.size(); ++t) rp[t + 1] = rp[t] + ph3::res_slices(table[c.idx[t]]);
+        o_res[ci] = put(rp);
+        c.res_total = rp.back();
+        c.partial_base = partials;
+        partials += c.res_total;
     }
     if (ih.size() > 5 * (size_t)b->cap + 1 || lh.size() > (size_t)b->cap + kMaxClasses) {
         set_error("ph3_batch_set: index tables exceed the batch's arenas");
@@ -854,13 +850,10 @@ extern "C" int ph3_batch_set(ph3_batch* b, const ph3_solve_desc* descs, int n, v
         b->n_small[st] = (int)small[st].size();
         b->d_small[st] = b->n_small[st] ? b->d_iarena + o_small[st] : nullptr;
     }
-    b->n_large = (int)large.size();
-    b->d_large = b->n_large ? b->d_iarena + o_large : nullptr;
-    b->d_res_prefix = b->d_iarena + o_res;
-    b->res_total = res_prefix.back();
     for (size_t ci = 0; ci < classes.size(); ++ci) {
         classes[ci].d_list = b->d_iarena + o_list[ci];
         classes[ci].d_prefix = b->d_llarena + o_prefix[ci];
+        classes[ci].d_res_prefix = b->d_iarena + o_res[ci];
     }
     b->classes = std::move(classes);
     b->n = n;
@@ -939,7 +932,7 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
     // graph: two parallel branches). Its few long-running workgroups then no longer keep the
     // tier-2 passes waiting.
     const bool any_small = b->n_small[0] + b->n_small[1] + b->n_small[2] > 0;
-    const bool fork = any_small && b->n_large > 0;
+    const bool fork = any_small && !b->classes.empty();
     hipStream_t s1 = fork ? b->side : s;
     if (fork) {
         PH3_CHECK(hipEventRecord(b->fork, s));
@@ -959,24 +952,26 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
         PH3_LAUNCH_CHECK("batch tier-1 launch");
     }
     if (fork) PH3_CHECK(hipEventRecord(b->join, s1));
-    // tier 2: segments of PH3_BATCH_CHECK varlen passes, then advance / residual / check
-    if (b->n_large) {
-        const unsigned gl = (unsigned)((b->n_large + 255) / 256);
-        const int rt = b->res_total > 0 ? b->res_total : 1;
-        const unsigned gr = (unsigned)(rt < 2 * ph3::kPassGroups ? rt : 2 * ph3::kPassGroups);
-        // Classes touch disjoint solves, so within a segment each runs its chain of passes on
-        // its own stream (the first on `stream`), forked and joined with events: a round then
-        // takes about the longest chain instead of the sum of all of them.
-        const int nc = (int)b->classes.size();
+    // tier 2: per class, segments of PH3_BATCH_CHECK sweeps in k-step varlen passes, each
+    // followed by the class's residual and check (advance / convergence test). Classes touch
+    // disjoint solves, so each runs its whole round on its own stream (the first on `stream`),
+    // forked and joined once per round with events: the classes' chains, residuals and checks
+    // overlap, and a round takes about the longest chain.
+    const int nc = (int)b->classes.size();
+    if (nc > 0) {
         const int nside = nc - 1 < ph3::kClassStreams ? nc - 1 : ph3::kClassStreams;
-        for (int seg = 0; seg < segs; ++seg) {
-            if (nside > 0) {
-                PH3_CHECK(hipEventRecord(b->seg, s));
-                for (int i = 0; i < nside; ++i) PH3_CHECK(hipStreamWaitEvent(b->cls[i], b->seg, 0));
-            }
-            for (int ci = 0; ci < nc; ++ci) {
-                const auto& c = b->classes[ci];
-                hipStream_t sc = ci == 0 ? s : b->cls[(ci - 1) % nside];
+        if (nside > 0) {
+            PH3_CHECK(hipEventRecord(b->seg, s));
+            for (int i = 0; i < nside; ++i) PH3_CHECK(hipStreamWaitEvent(b->cls[i], b->seg, 0));
+        }
+        for (int ci = 0; ci < nc; ++ci) {
+            const auto& c = b->classes[ci];
+            hipStream_t sc = ci == 0 ? s : b->cls[(ci - 1) % nside];
+            const int n = (int)c.idx.size();
+            const unsigned gl = (unsigned)((n + 255) / 256);
+            const unsigned gr = (unsigned)(c.res_total < 2 * ph3::kPassGroups ? c.res_total : 2 * ph3::kPassGroups);
+            double* part = b->d_partial + c.partial_base;
+            for (int seg = 0; seg < segs; ++seg) {
                 for (int p = 0; p < PH3_BATCH_CHECK / c.kpass; ++p) {  // k-step passes: fewer
                     switch (c.storage) {
                         case ph3::kF32: pass_class<float>(c, b, p, sc); break;
@@ -985,24 +980,21 @@ extern "C" int ph3_batch_round(ph3_batch* b, int sweeps, void* stream) {
                     }
                     PH3_LAUNCH_CHECK("batch tier-2 pass");
                 }
-            }
-            for (int i = 0; i < nside; ++i) {
-                PH3_CHECK(hipEventRecord(b->cls_join[i], b->cls[i]));
-                PH3_CHECK(hipStreamWaitEvent(s, b->cls_join[i], 0));
-            }
-            bool st_used[3] = {false, false, false};
-            for (const auto& c : b->classes) st_used[c.storage] = true;
-            for (int st = 0; st < 3; ++st) {
-                if (!st_used[st] || b->res_total == 0) continue;   // no tier-2 Jacobi solve
-                switch (st) {
-                    case ph3::kF32: ph3::batch_residual<float><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial); break;
-                    case ph3::kF16: ph3::batch_residual<_Float16><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial); break;
-                    default: ph3::batch_residual<ph3::bf16_t><<<gr, ph3::kWG, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial); break;
+                if (c.res_total > 0) {   // the class has Jacobi solves
+                    switch (c.storage) {
+                        case ph3::kF32: ph3::batch_residual<float><<<gr, ph3::kWG, 0, sc>>>(b->d_desc, c.d_list, c.d_res_prefix, n, part); break;
+                        case ph3::kF16: ph3::batch_residual<_Float16><<<gr, ph3::kWG, 0, sc>>>(b->d_desc, c.d_list, c.d_res_prefix, n, part); break;
+                        default: ph3::batch_residual<ph3::bf16_t><<<gr, ph3::kWG, 0, sc>>>(b->d_desc, c.d_list, c.d_res_prefix, n, part); break;
+                    }
+                    PH3_LAUNCH_CHECK("batch residual");
                 }
-                PH3_LAUNCH_CHECK("batch residual");
+                ph3::batch_check<<<gl, 256, 0, sc>>>(b->d_desc, c.d_list, c.d_res_prefix, n, part);
+                PH3_LAUNCH_CHECK("batch check");
             }
-            ph3::batch_check<<<gl, 256, 0, s>>>(b->d_desc, b->d_large, b->d_res_prefix, b->n_large, b->d_partial);
-            PH3_LAUNCH_CHECK("batch check");
+        }
+        for (int i = 0; i < nside; ++i) {
+            PH3_CHECK(hipEventRecord(b->cls_join[i], b->cls[i]));
+            PH3_CHECK(hipStreamWaitEvent(s, b->cls_join[i], 0));
         }
     }
     if (fork) PH3_CHECK(hipStreamWaitEvent(s, b->join, 0));

@@ -68,7 +68,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     const int lx = vec_col<V>(gx, nx, P);           // vector-aligned: nx % V == 0, gx % V == 0
     const size_t plane = (size_t)nx * ny;
 
-    size_t roff[R];
+    uint32_t roff[R];          // row offsets in a plane (a plane has fewer than 2^31 sites)
     uint32_t fixmask = 0;      // bit r*V+e: site (row r, element e) is a fixed Dirichlet site
     uint32_t wmask = 0;        // bit r*V+e: site is written (inside output region and grid)
     // Lanes are whole aligned V-site vectors and the row conditions are uniform per row, so a
@@ -77,7 +77,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int gy = gy0 + r;
-        roff[r] = (size_t)wrap_or_clamp(gy, ny, P) * nx;
+        roff[r] = (uint32_t)wrap_or_clamp(gy, ny, P) * (uint32_t)nx;
         const int row = wave * R + r;
         const bool row_out = row >= K && row < W * R - K && gy >= 0 && gy < ny;
 #pragma unroll
@@ -92,6 +92,22 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     }
     // Dirichlet: whether any lane of this wave holds a fixed site (uniform over the wave)
     const bool wave_fix = !P && __any(fixmask != 0u);
+    // float32 rows and the float32 source are read through a buffer resource over the plane
+    // (base in scalar registers), at byte offsets fixed for the whole march; the stores' byte
+    // offsets are fixed too (lanes and rows that never write hold the drop offset). pbytes: one
+    // plane of stored values. (Splitting each offset into a lane part and a wave-uniform row
+    // part in a scalar register cut the registers of the Dirichlet tb4 tile from 131 to 117,
+    // two workgroups per CU, but measured slower: 1024^3 periodic 1795 -> 1615 GLUPS,
+    // Dirichlet 1579 -> 1279; only 512^3 Dirichlet gained, 1124 -> 1287.)
+    const uint32_t pbytes = (uint32_t)(plane * sizeof(T));
+    uint32_t loff[R], soff[R];
+    constexpr uint32_t fullv = (1u << V) - 1u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        loff[r] = (roff[r] + (uint32_t)lx) * (uint32_t)sizeof(T);
+        soff[r] = ((wmask >> (r * V)) & fullv) == fullv ? (roff[r] + (uint32_t)gx) * (uint32_t)sizeof(T)
+                                                       : kDropOffset;
+    }
 
     // Plane queues in three register slots per level: at march step i (i = iz - z0 + K) level
     // s holds planes c-1, c, c+1 in slots i, i+1, i+2 (mod 3). A new value (the prefetched
@@ -128,11 +144,14 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     {
         const size_t zo = (size_t)wrap_or_clamp(z0 - K, nz, P) * plane;
         const size_t zb = (size_t)wrap_or_clamp(z0 - K - 1, nz, P) * plane;
+        const __amdgpu_buffer_rsrc_t ub = buffer_over(u + zo, pbytes);
+        const __amdgpu_buffer_rsrc_t bb =
+            buffer_over(b + (S ? zb : 0), S ? (uint32_t)(plane * sizeof(float)) : 0u);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
+            if constexpr (kDirect) load_vec_buf<V>(ub, loff[r], pref[r]);
             else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
-            if constexpr (S) load_vec<float, V>(b, zb + roff[r] + lx, bnext[r]);
+            if constexpr (S) load_vec_buf<V>(bb, (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
         }
         if constexpr (SM != kStoreMasked) {
 #pragma unroll
@@ -160,9 +179,10 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         }
         if (iz + 1 < z1 + K) {
             const size_t zo = (size_t)wrap_or_clamp(iz + 1, nz, P) * plane;
+            const __amdgpu_buffer_rsrc_t ub = buffer_over(u + zo, pbytes);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if constexpr (kDirect) load_vec<T, V>(u, zo + roff[r] + lx, pref[r]);
+                if constexpr (kDirect) load_vec_buf<V>(ub, loff[r], pref[r]);
                 else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
             }
         }
@@ -178,8 +198,9 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 #pragma unroll
                 for (int e = 0; e < V; ++e) bq[0][r][e] = bnext[r][e];
             const size_t zb = (size_t)wrap_or_clamp(iz, nz, P) * plane;
+            const __amdgpu_buffer_rsrc_t bb = buffer_over(b + zb, (uint32_t)(plane * sizeof(float)));
 #pragma unroll
-            for (int r = 0; r < R; ++r) load_vec<float, V>(b, zb + roff[r] + lx, bnext[r]);
+            for (int r = 0; r < R; ++r) load_vec_buf<V>(bb, (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
         }
         float res[R][V];
 #pragma unroll
@@ -254,13 +275,9 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             // fixed one, gets a 0-byte buffer); lanes outside the output region get the drop
             // offset
             const __amdgpu_buffer_rsrc_t ob = buffer_over(
-                o + (zok ? (size_t)c * plane : 0), zok ? (uint32_t)(plane * sizeof(T)) : 0u);
+                o + (zok ? (size_t)c * plane : 0), zok ? pbytes : 0u);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const bool w = ((wmask >> (r * V)) & full) == full;
-                store_vec_buf<T, V>(ob, w ? (uint32_t)((roff[r] + gx) * sizeof(T)) : kDropOffset,
-                                    res[r]);
-            }
+            for (int r = 0; r < R; ++r) store_vec_buf<T, V>(ob, soff[r], res[r]);
         }
     }
     }

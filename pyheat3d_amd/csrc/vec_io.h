@@ -61,7 +61,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void* p, uin
 
 template <typename T, int V>
 __device__ __forceinline__ void store_vec_buf(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes,
-                                              const float (&v)[V]) {
+                                              const float (&v)[V], uint32_t soff_bytes = 0) {
+    const int so = (int)soff_bytes;
     constexpr int B = V * (int)sizeof(T);
     static_assert(B == 2 || B == 4 || B == 8 || B == 16, "buffer stores of 2-16 bytes");
     T tmp[V];
@@ -70,19 +71,44 @@ __device__ __forceinline__ void store_vec_buf(__amdgpu_buffer_rsrc_t r, uint32_t
     if constexpr (B == 2) {
         unsigned short x;
         __builtin_memcpy(&x, tmp, 2);
-        __builtin_amdgcn_raw_buffer_store_b16(x, r, off_bytes, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(x, r, off_bytes, so, 0);
     } else if constexpr (B == 4) {
         unsigned int x;
         __builtin_memcpy(&x, tmp, 4);
-        __builtin_amdgcn_raw_buffer_store_b32(x, r, off_bytes, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(x, r, off_bytes, so, 0);
     } else if constexpr (B == 8) {
         u32x2_t x;
         __builtin_memcpy(&x, tmp, 8);
-        __builtin_amdgcn_raw_buffer_store_b64(x, r, off_bytes, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(x, r, off_bytes, so, 0);
     } else {
         u32x4_t x;
         __builtin_memcpy(&x, tmp, 16);
-        __builtin_amdgcn_raw_buffer_store_b128(x, r, off_bytes, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(x, r, off_bytes, so, 0);
+    }
+}
+
+// A raw buffer load of V float32 values at byte offset off_bytes (4 * V bytes, aligned to that):
+// the base address stays in scalar registers and each lane keeps a 32-bit offset, instead of a
+// 64-bit address per load. The loaded vector is copied out whole: extracting its elements one
+// by one and bit-casting each made this compiler shrink the load to one dword and copy that
+// dword into every element (checked in the gfx950 assembly).
+// soff_bytes: a wave-uniform part of the offset (scalar register), added by the hardware.
+template <int V>
+__device__ __forceinline__ void load_vec_buf(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes,
+                                             float (&out)[V], uint32_t soff_bytes = 0) {
+    static_assert(V == 1 || V == 2 || V == 4 || V == 8, "buffer loads of 1, 2, 4 or 8 floats");
+    const int so = (int)soff_bytes;
+    if constexpr (V == 1) {
+        out[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, so, 0));
+    } else if constexpr (V == 2) {
+        const u32x2_t x = __builtin_amdgcn_raw_buffer_load_b64(r, off_bytes, so, 0);
+        __builtin_memcpy(out, &x, 8);
+    } else {
+#pragma unroll
+        for (int h = 0; h < V / 4; ++h) {
+            const u32x4_t x = __builtin_amdgcn_raw_buffer_load_b128(r, off_bytes + 16 * h, so, 0);
+            __builtin_memcpy(out + 4 * h, &x, 16);
+        }
     }
 }
 
