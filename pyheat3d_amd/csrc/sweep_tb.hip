@@ -35,14 +35,14 @@ tb3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
         u, o, b, nz, ny, nx, c0, c1, zch, L % tiles_x, rest % tiles_y, rest / tiles_y);
 }
 
-template <typename T, bool P, bool S, int V, int K, int W>
+template <typename T, bool P, bool S, int V, int K, int W, bool EW = false>
 __global__ void __launch_bounds__(64 * W)
 tb2d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
      int ny, int nx, float c0, float c1, int ych, int tiles_x, int remap)
 {
     const int L = tb_logical_tile(remap);
-    tb2d_tile<T, P, S, V, K, kStoreAll>(u, o, b, ny, nx, c0, c1, ych,
-                                        (L % tiles_x) * W + (int)(threadIdx.x >> 6), L / tiles_x);
+    tb2d_tile<T, P, S, V, K, kStoreAll, EW>(u, o, b, ny, nx, c0, c1, ych,
+                                            (L % tiles_x) * W + (int)(threadIdx.x >> 6), L / tiles_x);
 }
 
 // ---- launch ---------------------------------------------------------------------------------
@@ -115,7 +115,7 @@ static int tb3_launch(const SweepArgs& a) {
     return PH3_OK;
 }
 
-template <typename T, bool P, bool S, int V, int K, int W>
+template <typename T, bool P, bool S, int V, int K, int W, bool EW = false>
 static int tb2_launch(const SweepArgs& a) {
     using G = TbGeom<V, K>;
     const int strips = (a.nx + G::OX - 1) / G::OX;
@@ -123,14 +123,14 @@ static int tb2_launch(const SweepArgs& a) {
     int ych = a.tile.zch;
     if (ych <= 0) {
         ych = tb_chunk(a.ny, tiles_x, K, 0);
-        const long res = cached_resident<tb2d<T, P, S, V, K, W>>(64 * W);
+        const long res = cached_resident<tb2d<T, P, S, V, K, W, EW>>(64 * W);
         if ((long)tiles_x * ((a.ny + ych - 1) / ych) < res)   // fewer workgroups than fit
             ych = fill_chunk(a.ny, tiles_x, res, ych, 2 * K, 2 * K);   // (1 row: 2048^2 tb4 -9%)
     }
     const long groups = (long)tiles_x * ((a.ny + ych - 1) / ych);
     if (groups > 0x7fffffffL) { ph3host::set_error("tb2d: grid too large"); return PH3_EINVAL; }
     if (planned(a, PH3_TB, K, V, 0, W, ych, 64 * V, 1, G::OX, 1, 64 * W, groups)) return PH3_OK;
-    tb2d<T, P, S, V, K, W><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
+    tb2d<T, P, S, V, K, W, EW><<<(unsigned)groups, 64 * W, 0, a.stream>>>(
         (const T*)a.u, (T*)a.o, a.b, a.ny, a.nx, a.c0, a.c1, ych, tiles_x,
         groups % 8 == 0 ? 1 : 0);
     PH3_LAUNCH_CHECK("tb2d launch");
@@ -163,9 +163,17 @@ static int tb2_shape(const SweepArgs& a) {
     // Default lane width: the k float32 levels of a lane's ring grow with it (k = 8: 58 VGPRs
     // at 2 sites, 114 at 4, i.e. half the resident waves), so k = 8 takes 2 sites, others 4.
     int v = a.tile.vec ? a.tile.vec : tb_fit_vec<K>(tb_pick_vec(a, (int)sizeof(T), K >= 8 ? 2 : 4), a.nx);
-    if (a.nx % v) { ph3host::set_error("tb2d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
     const int w = a.tile.waves ? a.tile.waves : 4;
     if (w != 4) { ph3host::set_error("tb2d: waves %d not built (4)", w); return PH3_EINVAL; }
+    // float32 rows whose width or alignment allows no vector lanes: 2-site lanes loading and
+    // storing site by site, where a 1-site strip would write fewer than 90% of the columns a
+    // 2-site one writes (k = 8: 48 of 64 vs 112 of 128)
+    constexpr bool kEwPays = TbGeom<1, K>::OX * 128 * 10 < 9 * TbGeom<2, K>::OX * 64;
+    if constexpr (sizeof(T) == 4 && kEwPays) {
+        if (!a.tile.vec && v == 1 && tb_fit_vec<K>(2, a.nx) == 2)
+            return tb2_launch<T, P, S, 2, K, 4, true>(a);
+    }
+    if (a.nx % v) { ph3host::set_error("tb2d: vec %d does not divide nx %d", v, a.nx); return PH3_EINVAL; }
     switch (v) {
         case 1: return tb2_launch<T, P, S, 1, K, 4>(a);
         case 2: return tb2_launch<T, P, S, 2, K, 4>(a);

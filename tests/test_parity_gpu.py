@@ -284,6 +284,30 @@ def test_tb2d_lane_vectors(key):
                     assert np.array_equal(u, ref), (key, shape, bc, vec, nr.rel_err(u, ref))
 
 
+def test_tb8_odd_widths_take_site_by_site_two_site_lanes():
+    """2D tb8 on widths that forbid vector lanes (odd, and 4-byte aligned only) runs 2-site
+    lanes loading and storing site by site (the resolved launch says vec 2) and equals the
+    oracle bit for bit, both BCs, with and without a source, on several strips and chunks."""
+    _need("hip-cdna4/tb8", 2)
+    rng = np.random.default_rng(31)
+    for shape in ((131, 149), (70, 65), (300, 257), (41, 1001)):
+        u0 = rng.random(shape, dtype=np.float32)
+        b = rng.random(shape, dtype=np.float32) * np.float32(1e-3)
+        for bc in ("dirichlet", "periodic"):
+            st = heat_stencil(2, 0.2, bc)
+            for src in (None, b):
+                ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 17, "float32", 8, src)
+                s = make_stepper("hip-cdna4/tb8", u0, st, src)
+                print(f"[gpu] tb8 {shape} {bc} src={src is not None} plan={s.tile}", flush=True)
+                try:
+                    assert s.tile["vec"] == 2, s.tile
+                    s.step(17)
+                    u = s.get()
+                finally:
+                    s.close()
+                assert np.array_equal(u, ref), (shape, bc, src is not None, nr.rel_err(u, ref))
+
+
 @pytest.mark.parametrize("bc", ["periodic", "dirichlet"])
 def test_large_grid_every_3d_rung_matches_oracle(bc):
     """512^3, 12 steps (a multiple of every k): every 3D rung bit for bit against the scalar C
