@@ -409,7 +409,8 @@ def serve_benchmark(n: int = 1024, rate: float = 500.0, seed: int = 2024,
                         f"50% Jacobi Laplace to 1e-6 (sweep cap {jacobi_cap}), 60% 2D N in "
                         f"{list(n2)}, 40% 3D N in {list(n3)}, {storage}, seed {seed}, HBM budget "
                         f"{budget_gb:g} GB",
-            "rounds": st.rounds, "max_resident": st.max_resident,
+            "rounds": st.rounds, "table_updates": st.table_updates,
+            "table_update_s": round(st.table_update_s, 3), "max_resident": st.max_resident,
             "max_resident_GB": st.max_resident_bytes / 1e9,
             **outcome_stats(reqs, res, st.seconds),
             "statuses": {s_: sum(r.status == s_ for r in res) for s_ in ("done", "stagnated", "max_sweeps")},

@@ -49,6 +49,8 @@ class ServeStats:
     max_resident_bytes: int
     rejected: int
     latencies_ms: list[float] = field(default_factory=list, repr=False)
+    table_updates: int = 0        # uploads of the solve table (admissions / retirements)
+    table_update_s: float = 0.0   # host time of those uploads and the round-graph re-records
 
 
 class Server:
@@ -90,7 +92,8 @@ class Server:
         used = 0
         graph = None
         dirty = True
-        rounds = max_res = max_bytes = 0
+        rounds = max_res = max_bytes = updates = 0
+        t_update = 0.0
         t0 = time.perf_counter()
         t_last = t0
         try:
@@ -115,6 +118,7 @@ class Server:
                 max_bytes = max(max_bytes, used)
                 # 2. table upload and round graph
                 if dirty:
+                    tu = time.perf_counter()
                     self.stream.sync()
                     # the round graph was recorded against the previous table: destroy it
                     # before the table changes under it
@@ -125,6 +129,8 @@ class Server:
                     if self.use_graph:
                         graph = _hip.Graph(self.stream, lambda: self.batch.round(self.round_sweeps))
                     dirty = False
+                    updates += 1
+                    t_update += time.perf_counter() - tu
                 # 3. one round
                 if graph is not None:
                     graph.launch(self.stream)
@@ -165,5 +171,6 @@ class Server:
         lat = np.array([results[r.rid].seconds for r in requests]) * 1e3
         stats = ServeStats(len(requests), total, len(requests) / total,
                            float(np.percentile(lat, 50)), float(np.percentile(lat, 99)),
-                           float(lat.max()), rounds, max_res, max_bytes, 0, lat.tolist())
+                           float(lat.max()), rounds, max_res, max_bytes, 0, lat.tolist(),
+                           updates, t_update)
         return [results[r.rid] for r in requests], stats
