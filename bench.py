@@ -25,7 +25,6 @@ import argparse
 import json
 import math
 import os
-import platform
 import sys
 import time
 from pathlib import Path
@@ -125,6 +124,7 @@ def cpu_baseline(n: int, min_time: float) -> dict:
     """NumPy oracle (periodic slicing stepper, one core) on an n^3 sample of the workload."""
     from oracle import numpy_ref as nr
     from pyheat3d_amd.analytic import sine_modes_periodic
+    from pyheat3d_amd.benchmark import cpu_model
     from pyheat3d_amd.problem import heat_stencil
 
     u = sine_modes_periodic(n, 3)
@@ -139,8 +139,8 @@ def cpu_baseline(n: int, min_time: float) -> dict:
         steps += 1
     return {"value": n ** 3 * steps / t / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
             "sample": f"NumPy oracle periodic slicing stepper, {n}^3 fp32, {steps} steps "
-                      f"({t:.1f} s) on one core of {platform.processor() or 'host'} "
-                      f"({os.cpu_count()} cores visible)"}
+                      f"({t:.1f} s) on one core of {cpu_model() or 'the host'} "
+                      f"({os.cpu_count()} cores visible)", "cpu_model": cpu_model()}
 
 
 def torch_eager_child(n: int, steps: int, warmup: int) -> int:
