@@ -178,9 +178,11 @@ int ph3_graph_destroy(void* graph_exec);
  * Tier 1 (grids of at most PH3_BATCH_SMALL_SITES sites: 64^2, 16^3): one workgroup owns the
  * grid and runs its sweeps back to back with workgroup barriers; a persistent kernel hands
  * grids to workgroups through an atomic counter. Larger grids advance faster as tier-2 tiles
- * (a 181^2 grid: ~40 us per sweep on one workgroup). Tier 2 (larger grids): each pass is one varlen launch
- * whose workgroups loop over the z-march tiles of every active large grid (a per-grid tile
- * table), followed at each check point by a deterministic per-grid residual.
+ * (a 181^2 grid: ~40 us per sweep on one workgroup). Tier 2 (larger grids), per class of solves
+ * (storage, dims, boundary, source, steps per pass, tile shape): each pass is one varlen launch
+ * over the tiles of every active grid of the class (one workgroup per tile, or one wave per 2D
+ * strip; a per-class tile prefix table), followed at each check point by a deterministic
+ * per-grid residual.
  * ------------------------------------------------------------------------------------------ */
 #define PH3_BATCH_CHECK 64
 #define PH3_BATCH_SMALL_SITES 4096
@@ -206,8 +208,9 @@ typedef struct ph3_solve_desc {
                                     planes only. 2D 1/2: 1/2-site lanes; 3/4: the same with a store
                                     on every step; 5: shape 4 with each site loaded and stored
                                     on its own (any width). Shapes 2 and 4 need an even width
-                                    above 64 and 8-byte aligned buffers (else the 1-site shape).
-                                    Never changes the result. */
+                                    above 64 and 8-byte aligned buffers (else 2D shape 4 runs as
+                                    5, the others as the 1-site shape). Never changes the
+                                    result. */
     long long steps;             /* mode 0: steps; mode 1: sweep cap */
     double tol;                  /* mode 1: target ||r|| / ||r0|| */
     double log_rate;             /* mode 1: ln(Jacobi spectral radius) * PH3_BATCH_CHECK (< 0) */
@@ -238,9 +241,9 @@ int ph3_batch_set(ph3_batch* batch, const ph3_solve_desc* descs, int n, void* st
 /* Enqueue one round: every running solve advances by up to `sweeps` steps (rounded up to a
  * multiple of PH3_BATCH_CHECK for mode 1), stopping early when finished. Asynchronous and
  * capturable into a graph (the launch shapes depend only on the table set last). Tier-1 work,
- * and the passes of every tier-2 class after the first, run on streams of the batch, forked
- * from and joined back to `stream` with events, so all of it is ordered on `stream` as seen by
- * the caller. */
+ * and the round of every tier-2 class after the first (its passes, residuals and checks), run
+ * on streams of the batch, forked from and joined back to `stream` with events once per round,
+ * so all of it is ordered on `stream` as seen by the caller. */
 int ph3_batch_round(ph3_batch* batch, int sweeps, void* stream);
 /* Copy the solve table back to descs[0..n) (host). Synchronous on `stream`. */
 int ph3_batch_get(ph3_batch* batch, ph3_solve_desc* descs, int n, void* stream);
