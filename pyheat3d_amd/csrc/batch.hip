@@ -5,11 +5,13 @@
 // sweeps back to back, separated only by workgroup barriers (the grid stays in L2); the FP64
 // residual is a workgroup reduction in a fixed order. A persistent kernel hands grids to
 // workgroups through an atomic counter, so a single launch advances every small grid.
-// Tier 2, larger grids: one varlen launch per pass whose workgroups loop over the z-march (3D)
-// or y-march (2D) tiles of every running grid of one (storage, dims, bc, source) class, found by
-// binary search in a per-class tile prefix table. Every PH3_BATCH_CHECK passes two small
-// kernels compute the residual of each Jacobi grid in fixed slices, then advance the step
-// counters and apply the convergence / stagnation / cap test.
+// Tier 2, larger grids: one varlen launch per pass covering the tiles of every running grid of
+// one (storage, dims, bc, source, steps per pass, tile shape) class: the fused temporal-blocking
+// tiles (float32) or the z-march / y-march tiles (single steps), one workgroup per tile (one wave
+// per strip for 2D fused passes), each finding its grid by binary search in a per-class tile
+// prefix table. After every PH3_BATCH_CHECK sweeps of a class, two small kernels on the
+// class's stream compute the residual of each of its Jacobi grids in fixed slices, then advance
+// the step counters and apply the convergence / stagnation / cap test.
 // All per-site arithmetic is the canonical update of ph3_common.h, so a grid gets the same bits
 // whichever tier, tile or batch it runs in; tier and check points depend only on the grid.
 #include "march_tiles.h"
