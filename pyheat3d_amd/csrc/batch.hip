@@ -691,6 +691,14 @@ static int check_desc(const ph3_solve_desc& d, int i) {
         set_error("solve %d: steps / stall_window out of range", i);
         return PH3_EINVAL;
     }
+    // the fused tier-2 tiles read and write through buffer resources over one plane (3D) or row
+    // (2D) of float32 values (the source is float32 whatever the storage)
+    const size_t span = (d.dims == 3 ? (size_t)d.ny : 1) * (size_t)d.nx * sizeof(float);
+    if (span > ph3::kMaxBufferBytes) {
+        set_error("solve %d: a %s of %zu bytes exceeds the %zu-byte buffer window", i,
+                  d.dims == 3 ? "plane" : "row", span, (size_t)ph3::kMaxBufferBytes);
+        return PH3_EUNSUP;
+    }
     return PH3_OK;
 }
 

@@ -207,8 +207,9 @@ static int tb_k(const SweepArgs& a) {
 
 int launch_tb(const SweepArgs& a, int k) {
     if (a.tile.flags) { ph3host::set_error("tb: tile flags must be 0"); return PH3_EINVAL; }
-    // the tiles store through buffer resources over one plane (3D) or row (2D)
-    const size_t elem = a.storage == kF32 ? 4 : 2;
+    // the tiles store (and float32 tiles and sources load) through buffer resources over one
+    // plane (3D) or row (2D); a float32 source plane is 4 bytes a site whatever the storage
+    const size_t elem = a.storage == kF32 || a.b ? 4 : 2;
     const size_t span = (a.dims == 3 ? (size_t)a.ny : 1) * (size_t)a.nx * elem;
     if (span > kMaxBufferBytes) {
         ph3host::set_error("tb: a %s of %zu bytes exceeds the %zu-byte store window",

@@ -207,3 +207,30 @@ def test_every_tier2_tile_shape_and_chunk_policy_matches_oracle(tile):
         ref = cref.run_f32(splitmix_field(q.shape, q.seed, q.kind), st.c0, st.c1, q.bc, r.steps,
                            q.source)
         assert np.array_equal(r.u, ref), (tile, q.shape, q.bc, q.mode, nr.rel_err(r.u, ref))
+
+
+def test_set_rejects_planes_beyond_the_buffer_window():
+    """A tier-2 solve whose plane of float32 values exceeds the buffer resources' 2 GiB window
+    is refused by ph3_batch_set before anything touches its buffers (the pointers here are never
+    dereferenced), and the batch keeps working for a valid table afterwards."""
+    from pyheat3d_amd.batch import Batch
+
+    stream = _hip.Stream()
+    b = Batch(2, stream)
+    try:
+        d = _hip.SolveDesc()
+        d.dims, d.nz, d.ny, d.nx = 3, 3, 32768, 32768      # a 4 GiB float32 plane
+        d.storage, d.c0, d.c1, d.mode, d.steps = 0, 0.0, 1.0 / 6.0, 0, 4
+        d.buf[0], d.buf[1] = 0x100000, 0x200000
+        with pytest.raises(_hip.HipError, match="buffer window"):
+            b.set([d])
+        req = Request(shape=(40, 36, 70), mode="heat", steps=8, seed=3)
+        s = BatchSolver([req], tuned_tiles=False)
+        try:
+            s.run()
+            assert s.results()[0].status == "done"
+        finally:
+            s.close()
+    finally:
+        b.close()
+        stream.close()
