@@ -30,8 +30,10 @@ tb3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
 {
     const int L = tb_logical_tile(remap);
     const int rest = L / tiles_x;
-    // store modes measured per k at 1024^3 (see StoreMode)
-    tb3d_tile<T, P, S, V, R, W, K, K == 4 ? kStorePlanes : kStoreMasked>(
+    // store modes measured per k at 1024^3 (see StoreMode); stage pinning where it pays (see
+    // tb3d_tile): periodic k = 4 tiles of 2- and 4-site lanes
+    tb3d_tile<T, P, S, V, R, W, K, K == 4 ? kStorePlanes : kStoreMasked,
+              P && !S && K == 4 && V >= 2>(
         u, o, b, nz, ny, nx, c0, c1, zch, L % tiles_x, rest % tiles_y, rest / tiles_y);
 }
 

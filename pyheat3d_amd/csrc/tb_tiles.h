@@ -52,7 +52,19 @@ enum StoreMode : int { kStoreMasked = 0, kStoreAll = 1, kStorePlanes = 2 };
 // One overlapped tile (tx, ty) of z chunk cz, shared with the batched varlen kernels. Every
 // wave reaches every barrier (no early return). A caller running several tiles in a row must
 // put a barrier between them (the LDS slot is reused).
-template <typename T, bool P, bool S, int V, int R, int W, int K, int SM = kStoreMasked>
+//
+// PIN: keep each stage's arithmetic before the next stage's LDS exchange. The exchanges of a
+// plane step read only planes of the previous step, so without it the compiler hoists all k
+// exchanges (and their barriers) to the top of the step and runs the k stages' arithmetic
+// after them, holding the up / down rows of every stage at once. Pinned, the 2-site 4 x 8
+// periodic tile of k = 4 drops from 132 to 125 registers (two workgroups per CU instead of
+// one) and the 16-wave tiles of k = 4 stop spilling. Measured at 1024^3 periodic float32
+// (GLUPS, best chunk): (vec 2, rows 4, waves 8) 1654 -> 1930, (2, 4, 16) 1366 -> 1578,
+// (4, 2, 16) 799 -> 1231; bfloat16 (2, 4, 8) 1770 -> 2070. 1-site lanes lose 7-15% and k = 2,
+// 3 tiles up to 15% (no occupancy to gain, the exchange latency is exposed instead), so those
+// stay unpinned. Dirichlet tiles keep this order anyway (the per-stage select is a branch).
+template <typename T, bool P, bool S, int V, int R, int W, int K, int SM = kStoreMasked,
+          bool PIN = false>
 __device__ __forceinline__ void
 tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
           int nz, int ny, int nx, float c0, float c1, int zch, int tx, int ty, int cz)
@@ -257,6 +269,12 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
 #pragma unroll
                     for (int e = 0; e < V; ++e)
                         if (zfixed || ((fixmask >> (r * V + e)) & 1u)) dst[r][e] = ctr[r][e];
+            }
+            if constexpr (PIN) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int e = 0; e < V; ++e) asm volatile("" : "+v"(dst[r][e]));
             }
         }
         const int c = iz - K;
