@@ -23,6 +23,21 @@
 
 namespace ph3 {
 
+// Tiles that exchange the edge rows of all k stages of a plane step at once, behind one
+// barrier (X1 in tb3d_tile), chosen from a sweep of every built shape at 256^3-1024^3 float32
+// (and bfloat16 at 768^3 / 1024^3), both boundaries, with and without a source: the 16-wave
+// 2-site tiles of k >= 3 gain 4-27% (tb4 1024^3 periodic 1635 -> 2078 GLUPS, Dirichlet
+// 1673 -> 2017), every tile with a source 12-34%, 16-wave 1-site Dirichlet tiles of k >= 3
+// 7-12%; 1-site periodic tiles lose 10-20% and 8-wave tiles are within 2%. The 2 x k slots of
+// every wave's two edge rows must fit the LDS (k * waves * vec <= 160: 160 KiB).
+template <bool P, bool S, int V, int W, int K>
+constexpr bool tb3_one_exchange() {
+    if (K * W * V > 160) return false;
+    if (S) return true;
+    if (K >= 3 && W == 16 && V == 2) return true;
+    return !P && K >= 3 && W == 16 && V == 1;
+}
+
 template <typename T, bool P, bool S, int V, int R, int W, int K>
 __global__ void __launch_bounds__(64 * W)
 tb3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
@@ -33,7 +48,7 @@ tb3d(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
     // store modes measured per k at 1024^3 (see StoreMode); stage pinning where it pays (see
     // tb3d_tile): periodic k = 4 tiles of 2- and 4-site lanes
     tb3d_tile<T, P, S, V, R, W, K, K == 4 ? kStorePlanes : kStoreMasked,
-              P && !S && K == 4 && V >= 2>(
+              P && !S && K == 4 && V >= 2, tb3_one_exchange<P, S, V, W, K>()>(
         u, o, b, nz, ny, nx, c0, c1, zch, L % tiles_x, rest % tiles_y, rest / tiles_y);
 }
 

@@ -63,15 +63,20 @@ enum StoreMode : int { kStoreMasked = 0, kStoreAll = 1, kStorePlanes = 2 };
 // (4, 2, 16) 799 -> 1231; bfloat16 (2, 4, 8) 1770 -> 2070. 1-site lanes lose 7-15% and k = 2,
 // 3 tiles up to 15% (no occupancy to gain, the exchange latency is exposed instead), so those
 // stay unpinned. Dirichlet tiles keep this order anyway (the per-stage select is a branch).
+//
+// X1: exchange the edge rows of all k stages at the top of the plane step, behind one barrier
+// (k slots per parity), instead of one exchange and one barrier per stage. Each stage still
+// reads its own slot just before its arithmetic. Costs k times the LDS of one exchange.
 template <typename T, bool P, bool S, int V, int R, int W, int K, int SM = kStoreMasked,
-          bool PIN = false>
+          bool PIN = false, bool X1 = false>
 __device__ __forceinline__ void
 tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ b,
           int nz, int ny, int nx, float c0, float c1, int zch, int tx, int ty, int cz)
 {
     using G = TbGeom<V, K>;
     constexpr int OY = W * R - 2 * K;  // output rows per tile
-    __shared__ float edge[2][W][2][64 * V];
+    constexpr int NXS = X1 ? K : 1;   // exchange slots per parity
+    __shared__ float edge[2][NXS][W][2][64 * V];
     const int lane = threadIdx.x & 63;
     const int wave = wave_id();
     const int z0 = cz * zch, z1 = min(z0 + zch, nz);
@@ -174,7 +179,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             }
         }
     }
-    int par = 0;   // LDS slot parity, flipped once per stage
+    int par = 0;   // LDS slot parity, flipped once per stage (X1: once per plane step)
     for (int ib = z0 - K; ib < z1 + K; ib += 3) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -215,6 +220,16 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             for (int r = 0; r < R; ++r) load_vec_buf<V>(bb, (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
         }
         float res[R][V];
+        if constexpr (X1 && W > 1) {
+#pragma unroll
+            for (int s = 1; s <= K; ++s)
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    edge[par][s - 1][wave][0][lane * V + e] = q[s - 1][(k + 1) % 3][0][e];
+                    edge[par][s - 1][wave][1][lane * V + e] = q[s - 1][(k + 1) % 3][R - 1][e];
+                }
+            __syncthreads();
+        }
 #pragma unroll
         for (int s = 1; s <= K; ++s) {
             const int c = iz - s;                       // plane computed by this stage
@@ -226,12 +241,14 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             // rows of the neighbouring waves (level s-1, plane c)
             float up[V], dn[V];
             if (W > 1) {
+                if constexpr (!X1) {
 #pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    edge[par][wave][0][lane * V + e] = ctr[0][e];
-                    edge[par][wave][1][lane * V + e] = ctr[R - 1][e];
+                    for (int e = 0; e < V; ++e) {
+                        edge[par][0][wave][0][lane * V + e] = ctr[0][e];
+                        edge[par][0][wave][1][lane * V + e] = ctr[R - 1][e];
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
                 // the first and last wave read their own rows: the tile's outer k rows are
                 // never stored and a value there reaches only rows within the light cone of
                 // the error, so any finite value does; an unconditional read costs no select
@@ -239,10 +256,10 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                 const int wu = wave > 0 ? wave - 1 : wave, wd = wave < W - 1 ? wave + 1 : wave;
 #pragma unroll
                 for (int e = 0; e < V; ++e) {
-                    up[e] = edge[par][wu][1][lane * V + e];
-                    dn[e] = edge[par][wd][0][lane * V + e];
+                    up[e] = edge[par][X1 ? s - 1 : 0][wu][1][lane * V + e];
+                    dn[e] = edge[par][X1 ? s - 1 : 0][wd][0][lane * V + e];
                 }
-                par ^= 1;
+                if constexpr (!X1) par ^= 1;
             } else {
 #pragma unroll
                 for (int e = 0; e < V; ++e) { up[e] = ctr[0][e]; dn[e] = ctr[R - 1][e]; }
@@ -282,6 +299,7 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
                     for (int e = 0; e < V; ++e) asm volatile("" : "+v"(dst[r][e]));
             }
         }
+        if constexpr (X1) par ^= 1;
         const int c = iz - K;
         const bool zok = c >= z0 && c < z1 && (P || (c > 0 && c < nz - 1));
         constexpr uint32_t full = (1u << V) - 1u;
