@@ -116,12 +116,21 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     // part in a scalar register cut the registers of the Dirichlet tb4 tile from 131 to 117,
     // two workgroups per CU, but measured slower: 1024^3 periodic 1795 -> 1615 GLUPS,
     // Dirichlet 1579 -> 1279; only 512^3 Dirichlet gained, 1124 -> 1287.)
+    constexpr bool kDirect = sizeof(T) == 4;   // float32 storage: rows read through buffers
+    // Lanes and rows past the last site the tile's outputs depend on (k beyond its last output
+    // column / row; the grid's far edge cuts the last tile of a row or column short) load
+    // nothing: their float32 loads take the drop offset, which returns zeros and fetches no
+    // bytes. Those zeros reach only sites that depend on them, none of which is stored.
+    const int xneed = min(tx * G::OX + G::OX, nx) + K;   // first x site no output depends on
+    const int yneed = min(ty * OY + OY, ny) + K;
+    const bool lane_need = gx < xneed;
     const uint32_t pbytes = (uint32_t)(plane * sizeof(T));
     uint32_t loff[R], soff[R];
     constexpr uint32_t fullv = (1u << V) - 1u;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        loff[r] = (roff[r] + (uint32_t)lx) * (uint32_t)sizeof(T);
+        const bool need = !kDirect || (lane_need && gy0 + r < yneed);
+        loff[r] = need ? (roff[r] + (uint32_t)lx) * (uint32_t)sizeof(T) : kDropOffset;
         soff[r] = ((wmask >> (r * V)) & fullv) == fullv ? (roff[r] + (uint32_t)gx) * (uint32_t)sizeof(T)
                                                        : kDropOffset;
     }
@@ -134,7 +143,6 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
     float q[K][3][R][V];
     // the next plane, loaded one step ahead: float32 as values, 16-bit storage still packed
     // (a conversion right after the load would make the wave wait for it there)
-    constexpr bool kDirect = sizeof(T) == 4;
     float pref[R][V];
     Packed<T, V> prefp[R];
     // float32 source: at march step iz, bq[s-1] holds plane iz-s (used by stage s); the planes
@@ -168,7 +176,8 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
         for (int r = 0; r < R; ++r) {
             if constexpr (kDirect) load_vec_buf<V>(ub, loff[r], pref[r]);
             else prefp[r] = load_packed<T, V>(u, zo + roff[r] + lx);
-            if constexpr (S) load_vec_buf<V>(bb, (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
+            if constexpr (S)
+                load_vec_buf<V>(bb, kDirect ? loff[r] : (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
         }
         if constexpr (SM != kStoreMasked) {
 #pragma unroll
@@ -217,7 +226,8 @@ tb3d_tile(const T* __restrict__ u, T* __restrict__ o, const float* __restrict__ 
             const size_t zb = (size_t)wrap_or_clamp(iz, nz, P) * plane;
             const __amdgpu_buffer_rsrc_t bb = buffer_over(b + zb, (uint32_t)(plane * sizeof(float)));
 #pragma unroll
-            for (int r = 0; r < R; ++r) load_vec_buf<V>(bb, (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
+            for (int r = 0; r < R; ++r)
+                load_vec_buf<V>(bb, kDirect ? loff[r] : (roff[r] + (uint32_t)lx) * 4u, bnext[r]);
         }
         float res[R][V];
         if constexpr (X1 && W > 1) {
