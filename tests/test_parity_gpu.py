@@ -255,6 +255,35 @@ def test_tb3d_every_tile_shape(k):
                     assert np.array_equal(u, ref), (k, bc, vec, r, w, zch, nr.rel_err(u, ref))
 
 
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_tb3d_tiles_with_one_column_and_one_row_tails(k):
+    """Grids one lane vector wider and one row taller than a tile's output, so the last tile of
+    each row and column writes a single vector / row: lanes and rows past what that tile's
+    outputs depend on load nothing, and the sites next to that cut must still come out bitwise,
+    for every built shape, both BCs, with and without a source."""
+    rng = np.random.default_rng(30 + k)
+    for (vec, r, w) in TB3_SHAPES:
+        kv = -(-k // vec) * vec
+        ox, oy = 64 * vec - 2 * kv, w * r - 2 * k
+        shape = (9, oy + 1, ox + vec)
+        for bc in ("periodic", "dirichlet"):
+            u0 = rng.random(shape, dtype=np.float32)
+            b = rng.random(shape, dtype=np.float32) * np.float32(1e-3)
+            st = heat_stencil(3, 0.15, bc)
+            for src in (None, b):
+                ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 2 * k, b=src)
+                s = make_stepper(f"hip-cdna4/tb{k}", u0, st, src)
+                s.tile_override = _hip.Tile(r, 0, vec, w, 0)
+                print(f"[gpu] tb{k} tails {shape} {bc} src={src is not None} vec={vec} rows={r} "
+                      f"waves={w}", flush=True)
+                try:
+                    s.step(2 * k)
+                    u = s.get()
+                finally:
+                    s.close()
+                assert np.array_equal(u, ref), (k, shape, bc, vec, r, w, nr.rel_err(u, ref))
+
+
 @pytest.mark.parametrize("key", ["hip-cdna4/tb2", "hip-cdna4/tb3", "hip-cdna4/tb4",
                                  "hip-cdna4/tb8", "hip-cdna4/tb4-bf16"])
 def test_tb2d_lane_vectors(key):
