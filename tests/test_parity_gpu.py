@@ -255,6 +255,32 @@ def test_tb3d_every_tile_shape(k):
                     assert np.array_equal(u, ref), (k, bc, vec, r, w, zch, nr.rel_err(u, ref))
 
 
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+def test_tb4_16bit_every_tile_shape(storage):
+    """16-bit tb4 in every built 3D shape (the periodic 2- and 4-site tiles run with pinned
+    stages, the 16-wave 2-site and the sourced ones with one exchange per plane step), both
+    BCs, with and without a float32 source, bit for bit against the storage-rounding oracle."""
+    key = "hip-cdna4/tb4" + {"float16": "-fp16", "bfloat16": "-bf16"}[storage]
+    rng = np.random.default_rng(44)
+    for shape, bc in (((21, 67, 136), "periodic"), ((17, 40, 72), "dirichlet")):
+        u0 = rng.random(shape, dtype=np.float32)
+        b = rng.random(shape, dtype=np.float32) * np.float32(1e-3)
+        st = heat_stencil(3, 0.15, bc)
+        for src in (None, b):
+            ref = nr.emulated_reference(u0, st.c0, st.c1, bc, 8, storage, 4, src)
+            for (vec, r, w) in TB3_SHAPES:
+                s = make_stepper(key, u0, st, src)
+                s.tile_override = _hip.Tile(r, 0, vec, w, 0)
+                print(f"[gpu] {key} {shape} {bc} src={src is not None} vec={vec} rows={r} "
+                      f"waves={w}", flush=True)
+                try:
+                    s.step(8)
+                    u = s.get()
+                finally:
+                    s.close()
+                assert np.array_equal(u, ref), (storage, bc, vec, r, w, nr.rel_err(u, ref))
+
+
 @pytest.mark.parametrize("k", [2, 3, 4])
 def test_tb3d_tiles_with_one_column_and_one_row_tails(k):
     """Grids one lane vector wider and one row taller than a tile's output, so the last tile of
