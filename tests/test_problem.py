@@ -211,13 +211,14 @@ def test_bench_cpu_solve_seconds_runs_the_oracle():
 
 
 def test_batch_table_entries_are_valid():
-    """Batch entries of the tuning table: tile shape ids 0-4 for every size bucket of each fused
-    class, and chunk policies the library accepts (read back as ph3_batch_params arguments)."""
+    """Batch entries of the tuning table: a shape id the library builds (0 = its rule; 1-5 in
+    2D, 1-6 in 3D) for every size bucket of each fused class, and chunk policies the library
+    accepts (read back as ph3_batch_params arguments)."""
     from pyheat3d_amd import tune
 
     table = tune.load_table()
     tiles = {k: e for k, e in table.items() if k.startswith("batch/tile/")}
-    assert tiles and all(0 <= e["tile"] <= (4 if k.startswith("batch/tile/2d") else 6)
+    assert tiles and all(0 <= e["tile"] <= max(tune.BATCH_SHAPES[int(k.split("/")[2][0])])
                          for k, e in tiles.items())
     for (d, _k), (lo, hi) in tune.BATCH_CLASSES.items():
         for b in range(tune.bucket((lo,) * d), tune.bucket((hi,) * d) + 1):
